@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the rank-local gather-scatter on one MI355X.
+
+Default workload (the headline configuration): ogs_t::GatherScatter, fp64,
+Add, Sym, K=1 on the 48^3-element hex box at p=7 (56,623,104 local DOFs,
+38,272,753 global DOFs), ids from the structured numbering, values
+uniform[-1,1] from a fixed seed, all resident in HBM before timing.
+
+One step = one GatherScatter call (one kernel launch).  W untimed warm-ups,
+then K timed steps between synchronisations, timed with HIP events on the
+stream the kernel is launched on.  DOFS/s follows the reference's
+definition: NgatherGlobal / time per call (dawgsMain.cpp:207).
+
+Prints ONE JSON line on stdout; diagnostics go to stderr.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 10] [--nx 48] [--p 7] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GatherScatter DOFS/s + % HBM peak, fp64 48^3 p=7; batched/fp32 vs plain"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gs_f64_add_sym_48cubed_p7_traffic.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpu() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+def algorithmic_bytes(nnz: int, nrows: int, K: int, s: int) -> int:
+    """GatherScatter, Sym (gather list == scatter list): each value read and
+    written once, each colId and row start read once."""
+    return 2 * K * nnz * s + 4 * nnz + 4 * (nrows + 1)
+
+
+def cpu_baseline(csr, N, NgatherGlobal, budget_s=12.0, max_calls=20):
+    """The reference's host GatherScatter loop (ogsOperator.cpp:311-360,
+    restated in the oracle, OpenMP over rows) on the same CSR and workload."""
+    from oracle import oracle as O
+    ref = O.OracleCsr(csr["rowStartsT"], csr["colIdsT"], Ncols=N)
+    v = np.random.default_rng(1).uniform(-1, 1, N)
+    for _ in range(3):
+        ref.gather_scatter(v, 1, O.ADD, O.SYM)
+    t0 = time.perf_counter()
+    calls = 0
+    while calls < max_calls and (calls < 2 or time.perf_counter() - t0 < budget_s):
+        ref.gather_scatter(v, 1, O.ADD, O.SYM)
+        calls += 1
+    t = (time.perf_counter() - t0) / calls
+    return {
+        "value": NgatherGlobal / t,
+        "unit": "DOFS/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"full workload, {calls} host GatherScatter calls after 3 warm-ups, "
+                  f"{t * 1e3:.1f} ms/call, OpenMP threads={O.num_threads()}, host CPU: {host_cpu()}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--p", type=int, default=7)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        log("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
+            "rank-local gather-scatter has no multi-GPU path here (see DESIGN.md, 'Sharding').")
+        return 2
+
+    import torch
+    import dawgs_amd as D
+    from dawgs_amd.mesh import make_ids
+
+    if not torch.cuda.is_available():
+        log("no HIP device visible")
+        return 1
+    torch.cuda.set_device(0)
+    nx, p = args.nx, args.p
+
+    t0 = time.perf_counter()
+    ids = make_ids(nx, nx, nx, p)
+    N = len(ids)
+    ogs = D.ogs_t()
+    ogs.Setup(N, ids, D.Unsigned, D.Auto, False, False)
+    del ids
+    info = ogs.plan.info()
+    log(f"setup {time.perf_counter() - t0:.2f} s: N={N} NgatherGlobal={ogs.NgatherGlobal} "
+        f"nnz={ogs.nnzT} blocks={info['nblocksK1']} W={info['widthK1']} maxRow={info['maxRowT']}")
+
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    q = torch.empty(N, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0, generator=gen)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        ogs.GatherScatter(q, 1, D.Add, D.Sym)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        ogs.GatherScatter(q, 1, D.Add, D.Sym)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+
+    B = algorithmic_bytes(ogs.nnzT, ogs.NrowsT, 1, 8)
+    achieved = B / (ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and (nx, p) == (48, 7):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": B}
+    log(f"GPU: {ms * 1e3:.1f} us/call, {achieved:.0f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s), "
+        f"{ogs.NgatherGlobal / (ms * 1e-3):.4g} DOFS/s")
+
+    cpu = None
+    if not args.no_cpu:
+        csr = ogs.csr()
+        cpu = cpu_baseline(csr, N, ogs.NgatherGlobal)
+        log(f"CPU: {cpu['sample']}: {cpu['value']:.4g} DOFS/s")
+
+    out = {
+        "metric": METRIC,
+        "value": ogs.NgatherGlobal / (ms * 1e-3),
+        "unit": "DOFS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ogs GatherScatter fp64 Add Sym K=1 on a {nx}^3 hex mesh, p={p} "
+                               f"({N} local DOFs, {ogs.NgatherGlobal} global), structured numbering, "
+                               f"values uniform[-1,1]",
+                   "nx": nx, "p": p, "K": 1, "op": "Add", "trans": "Sym",
+                   "local_dofs": N, "global_dofs": int(ogs.NgatherGlobal), "nnz": ogs.nnzT},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

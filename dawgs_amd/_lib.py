@@ -1,0 +1,79 @@
+"""Loads libogs_hip.so and declares the C-ABI of include/ogs_hip.h.
+
+There is no fallback: if the library is missing, importing fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libogs_hip.so")
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64P = C.POINTER(C.c_int64)
+_I32P = C.POINTER(C.c_int32)
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in
+                ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1")]
+
+
+class OgsInfo(C.Structure):
+    _fields_ = [("N", C.c_int32), ("Ngather", C.c_int32), ("NlocalT", C.c_int32),
+                ("NhaloT", C.c_int32), ("NlocalP", C.c_int32), ("NhaloP", C.c_int32),
+                ("NgatherGlobal", C.c_int64),
+                ("kind", C.c_int32), ("unique", C.c_int32), ("gather_defined", C.c_int32),
+                ("NrowsN", C.c_int32), ("NrowsT", C.c_int32), ("nnzN", C.c_int32), ("nnzT", C.c_int32),
+                ("NrowBlocksN", C.c_int32), ("NrowBlocksT", C.c_int32)]
+
+
+# name -> (restype, argtypes); mirrors include/ogs_hip.h exactly
+SIGNATURES = {
+    "ogs_hip_error_string": (C.c_char_p, [C.c_int]),
+    "ogs_hip_plan_create": (C.c_int, [C.POINTER(_P), _I32, _I32, _I32, _P, _P, _P, _P, _P]),
+    "ogs_hip_plan_destroy": (C.c_int, [_P]),
+    "ogs_hip_gather_scatter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P]),
+    "ogs_hip_gather": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P, _P]),
+    "ogs_hip_scatter": (C.c_int, [_P, C.c_int, C.c_int, _I32, _P, _P, _P]),
+    "ogs_hip_extract": (C.c_int, [C.c_int, _I32, _I32, _P, _P, _P, _P]),
+    "ogs_hip_plan_info": (C.c_int, [_P, C.POINTER(PlanInfo)]),
+    "ogs_setup": (C.c_int, [C.POINTER(_P), _I32, _P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ogs_free": (C.c_int, [_P]),
+    "ogs_get_info": (C.c_int, [_P, C.POINTER(OgsInfo)]),
+    "ogs_get_csr": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "ogs_get_plan": (_P, [_P]),
+    "ogs_gather_scatter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P]),
+    "ogs_gather": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P, _P]),
+    "ogs_scatter": (C.c_int, [_P, C.c_int, C.c_int, _I32, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m dawgs_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class OgsError(RuntimeError):
+    def __init__(self, code: int, where: str = ""):
+        msg = lib().ogs_hip_error_string(code).decode()
+        super().__init__(f"{where}: {msg} (code {code})" if where else f"{msg} (code {code})")
+        self.code = code
+
+
+def check(rc: int, where: str = "") -> None:
+    if rc != 0:
+        raise OgsError(rc, where)

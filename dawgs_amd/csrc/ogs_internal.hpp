@@ -1,0 +1,46 @@
+// ogs_internal.hpp -- private layout of an operator plan and the kernel
+// tiling constants shared by the gfx950 kernels and their launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ogs_hip {
+
+constexpr int kBlock = 256;                                     // lanes per workgroup (4 waves of 64)
+constexpr int kCap = 2048;                                      // values staged in LDS per list per pass
+constexpr int kSlots = kCap / kBlock;                           // staged values per lane
+constexpr int kWMax = kCap;                                     // rows per block never exceed the widest W
+constexpr int kRowSlots = (kWMax + 1 + kBlock - 1) / kBlock;    // row starts per lane
+constexpr int kNumXcd = 8;                                      // MI355X: 8 XCDs x 32 CUs
+constexpr int kMaxParts = 7;                                    // partition widths kCap>>0 .. kCap>>6
+
+// One merge-path row partition: nblocks+1 entries {row, rowStartsT[row],
+// rowStartsN[row], 0}; block b covers rows [part[b].x, part[b+1].x).
+struct Partition {
+  int4 *d = nullptr;
+  int32_t nblocks = 0;
+  int32_t W = 0;        // keys (rows + nonzeros) per block
+  int32_t maxT = 0;     // largest T-list nonzero count of a block
+  int32_t maxN = 0;     // largest N-list nonzero count of a block
+  int32_t maxRows = 0;  // largest row count of a block (<= W)
+};
+
+}  // namespace ogs_hip
+
+// The opaque plan behind ogs_hip_plan_t.
+struct ogs_hip_plan_s {
+  int device = 0;
+  int32_t Ncols = 0, NrowsN = 0, NrowsT = 0, nnzN = 0, nnzT = 0;
+  int32_t maxRowN = 0, maxRowT = 0;
+  const int32_t *rsN = nullptr, *ciN = nullptr, *rsT = nullptr, *ciT = nullptr;
+  bool aliased = false;  // N lists are the T lists (Unsigned operator)
+  int nparts = 0;
+  ogs_hip::Partition partT[ogs_hip::kMaxParts];  // cut on T, rows [0, NrowsT)
+  ogs_hip::Partition partN[ogs_hip::kMaxParts];  // cut on N, rows [0, NrowsN)
+};
+
+namespace ogs_hip {
+using Plan = ogs_hip_plan_s;
+}

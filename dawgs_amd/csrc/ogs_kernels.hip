@@ -1,0 +1,825 @@
+// ogs_kernels.hip -- gfx950 gather-scatter kernels and the operator-layer
+// C-ABI (ogs_hip_*) declared in include/ogs_hip.h.
+//
+// What the kernels compute is fixed by the reference's host operator
+// (libs/ogs/ogsOperator.cpp:36-378): for every CSR row, reduce the row's
+// gather list in CSR order starting from the op's identity, then write the
+// result to the row's scatter list.  How they compute it is MI355X-first:
+//
+//  * Partition.  A plan cuts the rows into blocks by a merge-path rule: row r
+//    belongs to block floor((rowStarts[r] + r) / W).  Every block then holds
+//    at most W rows and at most W + Lmax - 1 nonzeros, so the per-block LDS
+//    staging never overflows, and the cut is computed on the device by
+//    binary search (no host pass over the CSR).  Blocks are contiguous row
+//    ranges, so each block's column set is a compact window of v.
+//  * Loads.  Index streams (colIds, row starts) are read coalesced, each lane
+//    holding a fixed number of slots; all slots' loads are issued before the
+//    first wait, so a wave has 8 independent gathers in flight.
+//  * Reduction.  Gathered values are staged in LDS in CSR order; one lane per
+//    row reduces its row sequentially, which keeps fp Add bit-exact with the
+//    host loop.  Integer Add/Mul wrap (unsigned arithmetic); Min/Max are the
+//    reference's compare-selects, so NaN inputs are skipped like on the host.
+//  * XCDs.  Workgroups are dispatched round-robin over the 8 XCDs; block ids
+//    are remapped so that each XCD walks one contiguous eighth of the rows
+//    and neighbouring blocks share that XCD's L2.
+//  * K fields.  Fields are interleaved (v[k + id*K]); all K fields of a block
+//    are done in one pass over its indices when they fit in LDS, else in
+//    field chunks, so indices are read once per block rather than once per
+//    field as in the reference (ogsKernels.okl:41, @outer(1) over K).
+//
+// No MFMA: nothing here is a dense contraction.  No fast-math: the build
+// uses -ffp-contract=off and keeps IEEE semantics.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cstddef>
+#include <cstdint>
+#include <new>
+
+#include "ogs_hip.h"
+#include "ogs_internal.hpp"
+
+namespace ogs_hip {
+
+// ---------------------------------------------------------------------------
+// Reduction ops: ogsOperator.cpp:36-55 (host functors) and ogsUtils.cpp:63-103
+// (device inits and OGS_OP macros).  Inits are +-numeric_limits<T>::max(),
+// not +-inf and not INT_MIN.
+template <typename T> struct Lim;
+template <> struct Lim<float> { static __device__ constexpr float max() { return FLT_MAX; } };
+template <> struct Lim<double> { static __device__ constexpr double max() { return DBL_MAX; } };
+template <> struct Lim<int32_t> { static __device__ constexpr int32_t max() { return INT32_MAX; } };
+template <> struct Lim<int64_t> { static __device__ constexpr int64_t max() { return INT64_MAX; } };
+
+template <typename T> __device__ __forceinline__ T wrap_add(T a, T b) { return a + b; }
+template <typename T> __device__ __forceinline__ T wrap_mul(T a, T b) { return a * b; }
+template <> __device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+template <> __device__ __forceinline__ int32_t wrap_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+template <> __device__ __forceinline__ int64_t wrap_add(int64_t a, int64_t b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
+template <> __device__ __forceinline__ int64_t wrap_mul(int64_t a, int64_t b) { return (int64_t)((uint64_t)a * (uint64_t)b); }
+
+template <typename T, int OP> struct Red;
+template <typename T> struct Red<T, OGS_ADD> {
+  static __device__ __forceinline__ T init() { return T(0); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a = wrap_add(a, b); }
+};
+template <typename T> struct Red<T, OGS_MUL> {
+  static __device__ __forceinline__ T init() { return T(1); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a = wrap_mul(a, b); }
+};
+template <typename T> struct Red<T, OGS_MAX> {
+  static __device__ __forceinline__ T init() { return -Lim<T>::max(); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a = (b > a) ? b : a; }
+};
+template <typename T> struct Red<T, OGS_MIN> {
+  static __device__ __forceinline__ T init() { return Lim<T>::max(); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a = (b < a) ? b : a; }
+};
+
+// Workgroup id -> block id such that XCD x (which receives workgroups
+// x, x+8, x+16, ...) processes one contiguous run of blocks.
+__device__ __forceinline__ int xcd_block(int wg, int nwg) {
+  const int xcd = wg % kNumXcd, loc = wg / kNumXcd;
+  const int q = nwg / kNumXcd, r = nwg % kNumXcd;
+  return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
+// Block extent shared by all kernels.  A partition entry is
+// {first row, rowStartsT[row], rowStartsN[row], 0}.
+struct Extent {
+  int r0, nr;  // rows [r0, r0+nr)
+  int g0, nG;  // gather list nonzeros [g0, g0+nG)
+  int s0, nS;  // scatter list nonzeros [s0, s0+nS)
+};
+
+__device__ __forceinline__ bool block_extent(const int4 *__restrict__ part, int nrows,
+                                             const int32_t *__restrict__ gS, int gUseN,
+                                             const int32_t *__restrict__ sS, int sUseN, Extent &e) {
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int4 p0 = part[b];
+  const int4 p1 = part[b + 1];
+  const int r1 = min(p1.x, nrows);
+  e.r0 = p0.x;
+  if (e.r0 >= r1) return false;
+  e.nr = r1 - e.r0;
+  const bool full = (r1 == p1.x);
+  e.g0 = gUseN ? p0.z : p0.y;
+  e.nG = (full ? (gUseN ? p1.z : p1.y) : gS[r1]) - e.g0;
+  e.s0 = sUseN ? p0.z : p0.y;
+  e.nS = (full ? (sUseN ? p1.z : p1.y) : sS[r1]) - e.s0;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// GatherScatter, K = 1.  SYM: the gather and scatter lists are the same
+// array (Sym, or an Unsigned operator), so colIds stay in registers between
+// the gather and the scatter and the scatter reuses the gather slots.
+template <typename T, int OP, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_gs_k1(const int4 *__restrict__ part, int nrows,
+                                                  const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
+                                                  const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
+                                                  T *__restrict__ q) {
+  __shared__ T gval[kCap];
+  __shared__ T sval[SYM ? 1 : kCap];
+  __shared__ int32_t rsG[kWMax + 1];
+  __shared__ int32_t rsS[SYM ? 1 : kWMax + 1];
+
+  Extent e;
+  if (!block_extent(part, nrows, gS, gUseN, sS, sUseN, e)) return;
+  const int tid = threadIdx.x;
+
+  int c[kSlots];
+  int cs[SYM ? 1 : kSlots];
+  int rg[kRowSlots];
+  int rss[SYM ? 1 : kRowSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int i = tid + j * kBlock;
+    c[j] = (i < e.nG) ? __builtin_nontemporal_load(gI + e.g0 + i) : 0;
+  }
+  if constexpr (!SYM) {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int i = tid + j * kBlock;
+      cs[j] = (i < e.nS) ? __builtin_nontemporal_load(sI + e.s0 + i) : 0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    rg[j] = (r <= e.nr) ? __builtin_nontemporal_load(gS + e.r0 + r) : 0;
+    if constexpr (!SYM) rss[j] = (r <= e.nr) ? __builtin_nontemporal_load(sS + e.r0 + r) : 0;
+  }
+  T x[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int i = tid + j * kBlock;
+    if (i < e.nG) x[j] = q[c[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    if (r <= e.nr) {
+      rsG[r] = rg[j] - e.g0;
+      if constexpr (!SYM) rsS[r] = rss[j] - e.s0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int i = tid + j * kBlock;
+    if (i < e.nG) gval[i] = x[j];
+  }
+  __syncthreads();
+
+  for (int r = tid; r < e.nr; r += kBlock) {
+    const int a = rsG[r], b = rsG[r + 1];
+    T acc = Red<T, OP>::init();
+    for (int i = a; i < b; ++i) Red<T, OP>::apply(acc, gval[i]);
+    if constexpr (SYM) {
+      for (int i = a; i < b; ++i) gval[i] = acc;
+    } else {
+      const int sa = rsS[r], sb = rsS[r + 1];
+      for (int i = sa; i < sb; ++i) sval[i] = acc;
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int i = tid + j * kBlock;
+    if constexpr (SYM) {
+      if (i < e.nG) q[c[j]] = gval[i];
+    } else {
+      if (i < e.nS) q[cs[j]] = sval[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GatherScatter, general K: fields [k0, k0+kw) per pass, kw*nnz <= kCap.
+// Flat index f = i*kw + k runs over (nonzero, field) pairs, so consecutive
+// lanes read consecutive fields of one node.
+template <typename T, int OP, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_gs_kv(const int4 *__restrict__ part, int nrows,
+                                                  const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
+                                                  const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
+                                                  int K, int kc, T *__restrict__ q) {
+  __shared__ T gval[kCap];
+  __shared__ T sval[SYM ? 1 : kCap];
+  __shared__ int32_t rsG[kWMax + 1];
+  __shared__ int32_t rsS[SYM ? 1 : kWMax + 1];
+
+  Extent e;
+  if (!block_extent(part, nrows, gS, gUseN, sS, sUseN, e)) return;
+  const int tid = threadIdx.x;
+
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    if (r <= e.nr) {
+      rsG[r] = gS[e.r0 + r] - e.g0;
+      if constexpr (!SYM) rsS[r] = sS[e.r0 + r] - e.s0;
+    }
+  }
+
+  for (int k0 = 0; k0 < K; k0 += kc) {
+    const int kw = min(kc, K - k0);
+    const int nvG = e.nG * kw;
+    size_t off[kSlots];
+    T x[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvG) {
+        const int i = f / kw;
+        off[j] = (size_t)gI[e.g0 + i] * K + k0 + (f - i * kw);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvG) x[j] = q[off[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvG) gval[f] = x[j];
+    }
+    __syncthreads();
+
+    for (int t = tid; t < e.nr * kw; t += kBlock) {
+      const int r = t / kw, k = t - r * kw;
+      const int a = rsG[r], b = rsG[r + 1];
+      T acc = Red<T, OP>::init();
+      for (int i = a; i < b; ++i) Red<T, OP>::apply(acc, gval[i * kw + k]);
+      if constexpr (SYM) {
+        for (int i = a; i < b; ++i) gval[i * kw + k] = acc;
+      } else {
+        const int sa = rsS[r], sb = rsS[r + 1];
+        for (int i = sa; i < sb; ++i) sval[i * kw + k] = acc;
+      }
+    }
+    __syncthreads();
+
+    if constexpr (SYM) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int f = tid + j * kBlock;
+        if (f < nvG) q[off[j]] = gval[f];
+      }
+    } else {
+      const int nvS = e.nS * kw;
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int f = tid + j * kBlock;
+        if (f < nvS) {
+          const int i = f / kw;
+          off[j] = (size_t)sI[e.s0 + i] * K + k0 + (f - i * kw);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int f = tid + j * kBlock;
+        if (f < nvS) q[off[j]] = sval[f];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather: gv[k + row*K] = OP over the row's list.  Rows [0, nrows).
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gather(const int4 *__restrict__ part, int nrows,
+                                                   const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
+                                                   int K, int kc, const T *__restrict__ q, T *__restrict__ gq) {
+  __shared__ T gval[kCap];
+  __shared__ int32_t rsG[kWMax + 1];
+
+  Extent e;
+  if (!block_extent(part, nrows, gS, gUseN, gS, gUseN, e)) return;
+  const int tid = threadIdx.x;
+
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    if (r <= e.nr) rsG[r] = gS[e.r0 + r] - e.g0;
+  }
+  for (int k0 = 0; k0 < K; k0 += kc) {
+    const int kw = min(kc, K - k0);
+    const int nvG = e.nG * kw;
+    size_t off[kSlots];
+    T x[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvG) {
+        const int i = f / kw;
+        off[j] = (size_t)gI[e.g0 + i] * K + k0 + (f - i * kw);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvG) x[j] = q[off[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvG) gval[f] = x[j];
+    }
+    __syncthreads();
+    for (int t = tid; t < e.nr * kw; t += kBlock) {
+      const int r = t / kw, k = t - r * kw;
+      const int a = rsG[r], b = rsG[r + 1];
+      T acc = Red<T, OP>::init();
+      for (int i = a; i < b; ++i) Red<T, OP>::apply(acc, gval[i * kw + k]);
+      gq[(size_t)(e.r0 + r) * K + k0 + k] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scatter: v[k + col*K] = gv[k + row*K].  Rows [0, nrows).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter(const int4 *__restrict__ part, int nrows,
+                                                    const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
+                                                    int K, int kc, const T *__restrict__ gq, T *__restrict__ q) {
+  __shared__ T sval[kCap];
+  __shared__ int32_t rsS[kWMax + 1];
+
+  Extent e;
+  if (!block_extent(part, nrows, sS, sUseN, sS, sUseN, e)) return;
+  const int tid = threadIdx.x;
+
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    if (r <= e.nr) rsS[r] = sS[e.r0 + r] - e.s0;
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += kc) {
+    const int kw = min(kc, K - k0);
+    const int ntask = e.nr * kw;
+    for (int t0 = 0; t0 < ntask; t0 += kBlock * kSlots) {
+      T x[kSlots];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int t = t0 + tid + j * kBlock;
+        if (t < ntask) {
+          const int r = t / kw, k = t - r * kw;
+          x[j] = gq[(size_t)(e.r0 + r) * K + k0 + k];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int t = t0 + tid + j * kBlock;
+        if (t < ntask) {
+          const int r = t / kw, k = t - r * kw;
+          const int a = rsS[r], b = rsS[r + 1];
+          for (int i = a; i < b; ++i) sval[i * kw + k] = x[j];
+        }
+      }
+    }
+    __syncthreads();
+    const int nvS = e.nS * kw;
+    size_t off[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvS) {
+        const int i = f / kw;
+        off[j] = (size_t)sI[e.s0 + i] * K + k0 + (f - i * kw);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nvS) q[off[j]] = sval[f];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// extract (ogsKernels.okl:167-177): gq[n] = q[n%K + ids[n/K]*K]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_extract(int64_t total, int K, const int32_t *__restrict__ ids,
+                                                    const T *__restrict__ q, T *__restrict__ gq) {
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < total; n += (int64_t)gridDim.x * kBlock) {
+    const int64_t g = n / K;
+    const int k = (int)(n - g * K);
+    gq[n] = q[(size_t)ids[g] * K + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plan-time kernels.
+
+// Longest row of a CSR and a validity check of its row starts and column ids.
+// out[0] = max row length, out[1] = error flags (1: decreasing row start,
+// 2: column id out of [0, ncols)).
+__global__ __launch_bounds__(kBlock) void k_scan_csr(const int32_t *__restrict__ rs, int nrows,
+                                                     const int32_t *__restrict__ ci, int nnz, int ncols,
+                                                     int32_t *__restrict__ out) {
+  __shared__ int32_t smax, sbad;
+  if (threadIdx.x == 0) { smax = 0; sbad = 0; }
+  __syncthreads();
+  int lmax = 0, bad = 0;
+  const int stride = gridDim.x * kBlock;
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < nrows; r += stride) {
+    const int len = rs[r + 1] - rs[r];
+    if (len < 0) bad |= 1;
+    lmax = max(lmax, len);
+  }
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nnz; i += stride) {
+    const int c = ci[i];
+    if (c < 0 || c >= ncols) bad |= 2;
+  }
+  atomicMax(&smax, lmax);
+  atomicOr(&sbad, bad);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicMax(&out[0], smax);
+    atomicOr(&out[1], sbad);
+  }
+}
+
+// Merge-path partition: entry b = first row r with key(r) >= b*W, where
+// key(r) = rowStartsX[r] + r (X = the list the partition is cut on).
+// Entry nblocks = nrows.  Entries carry both lists' starts.
+__global__ __launch_bounds__(kBlock) void k_partition(const int32_t *__restrict__ rsT, const int32_t *__restrict__ rsN,
+                                                      int cutOnN, int nrows, int W, int nblocks,
+                                                      int4 *__restrict__ part) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b > nblocks) return;
+  const int32_t *__restrict__ rs = cutOnN ? rsN : rsT;
+  int row;
+  if (b == nblocks) {
+    row = nrows;
+  } else {
+    const int64_t target = (int64_t)b * W;
+    int lo = 0, hi = nrows;  // answer in [0, nrows]
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if ((int64_t)(rs[mid] - rs[0]) + mid < target) lo = mid + 1;
+      else hi = mid;
+    }
+    row = lo;
+  }
+  part[b] = make_int4(row, rsT[row], rsN[row], 0);
+}
+
+// Per-block maxima of a partition: out[0] = max T nonzeros, out[1] = max N
+// nonzeros, out[2] = max rows.
+__global__ __launch_bounds__(kBlock) void k_part_max(const int4 *__restrict__ part, int nblocks,
+                                                     int32_t *__restrict__ out) {
+  __shared__ int32_t s[3];
+  if (threadIdx.x < 3) s[threadIdx.x] = 0;
+  __syncthreads();
+  int mT = 0, mN = 0, mR = 0;
+  for (int b = blockIdx.x * kBlock + threadIdx.x; b < nblocks; b += gridDim.x * kBlock) {
+    const int4 p0 = part[b], p1 = part[b + 1];
+    mT = max(mT, p1.y - p0.y);
+    mN = max(mN, p1.z - p0.z);
+    mR = max(mR, p1.x - p0.x);
+  }
+  atomicMax(&s[0], mT);
+  atomicMax(&s[1], mN);
+  atomicMax(&s[2], mR);
+  __syncthreads();
+  if (threadIdx.x < 3) atomicMax(&out[threadIdx.x], s[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers.
+
+static int grid_for(int64_t n, int cap = 65536) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+// Choose the partition and field chunk for a launch touching lists with
+// per-block maxima read by `maxOf`: prefer one pass over all K fields with
+// the widest blocks; otherwise the most fields per pass.
+template <class MaxOf>
+static const Partition *choose(const Partition *parts, int nparts, int K, MaxOf maxOf, int *kc) {
+  const Partition *best = nullptr;
+  int bestKc = 0;
+  for (int j = 0; j < nparts; ++j) {
+    const int m = maxOf(parts[j]);
+    const int fit = (m <= 0) ? K : kCap / m;
+    const int kcj = fit < K ? fit : K;
+    if (kcj <= 0) continue;
+    if (kcj == K) { *kc = K; return &parts[j]; }
+    if (kcj > bestKc) { bestKc = kcj; best = &parts[j]; }
+  }
+  *kc = bestKc;
+  return best;
+}
+
+template <typename T>
+static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
+  // list selection: ogsOperator.cpp:321-339
+  int nrows;
+  const int32_t *gS, *gI, *sS, *sI;
+  int gN, sN;
+  if (trans == OGS_TRANS) {
+    nrows = p->NrowsN; gS = p->rsT; gI = p->ciT; gN = 0; sS = p->rsN; sI = p->ciN; sN = 1;
+  } else if (trans == OGS_SYM) {
+    nrows = p->NrowsT; gS = p->rsT; gI = p->ciT; gN = 0; sS = p->rsT; sI = p->ciT; sN = 0;
+  } else {
+    nrows = p->NrowsT; gS = p->rsN; gI = p->ciN; gN = 1; sS = p->rsT; sI = p->ciT; sN = 0;
+  }
+  if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  const bool sym = (gI == sI && gS == sS);
+  int kc = 0;
+  const Partition *pt = choose(p->partT, p->nparts, K,
+                               [&](const Partition &x) { return gN || sN ? (x.maxT > x.maxN ? x.maxT : x.maxN) : x.maxT; }, &kc);
+  if (!pt) return OGS_E_ROW_TOO_LONG;
+  const dim3 grid(pt->nblocks), block(kBlock);
+  T *v = static_cast<T *>(d_v);
+#define OGS_GS_LAUNCH(OPV)                                                                                      \
+  if (K == 1) {                                                                                                 \
+    if (sym) hipLaunchKernelGGL((k_gs_k1<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, v); \
+    else hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, v);   \
+  } else {                                                                                                      \
+    if (sym) hipLaunchKernelGGL((k_gs_kv<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v); \
+    else hipLaunchKernelGGL((k_gs_kv<T, OPV, false>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v);   \
+  }
+  switch (op) {
+    case OGS_ADD: OGS_GS_LAUNCH(OGS_ADD) break;
+    case OGS_MUL: OGS_GS_LAUNCH(OGS_MUL) break;
+    case OGS_MAX: OGS_GS_LAUNCH(OGS_MAX) break;
+    case OGS_MIN: OGS_GS_LAUNCH(OGS_MIN) break;
+    default: return OGS_E_ARG;
+  }
+#undef OGS_GS_LAUNCH
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_gv, const void *d_v, hipStream_t st) {
+  // list selection: ogsOperator.cpp:71-79
+  const bool useN = (trans == OGS_NOTRANS);
+  const int nrows = useN ? p->NrowsN : p->NrowsT;
+  if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  const Partition *parts = (useN && !p->aliased) ? p->partN : p->partT;
+  int kc = 0;
+  const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
+  if (!pt) return OGS_E_ROW_TOO_LONG;
+  const int32_t *gS = useN ? p->rsN : p->rsT;
+  const int32_t *gI = useN ? p->ciN : p->ciT;
+  const int gN = useN ? 1 : 0;
+  const dim3 grid(pt->nblocks), block(kBlock);
+  const T *v = static_cast<const T *>(d_v);
+  T *gv = static_cast<T *>(d_gv);
+  switch (op) {
+    case OGS_ADD: hipLaunchKernelGGL((k_gather<T, OGS_ADD>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
+    case OGS_MUL: hipLaunchKernelGGL((k_gather<T, OGS_MUL>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
+    case OGS_MAX: hipLaunchKernelGGL((k_gather<T, OGS_MAX>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
+    case OGS_MIN: hipLaunchKernelGGL((k_gather<T, OGS_MIN>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, const void *d_gv, hipStream_t st) {
+  // list selection: ogsOperator.cpp:214-222
+  const bool useN = (trans == OGS_TRANS);
+  const int nrows = useN ? p->NrowsN : p->NrowsT;
+  if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  const Partition *parts = (useN && !p->aliased) ? p->partN : p->partT;
+  int kc = 0;
+  const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
+  if (!pt) return OGS_E_ROW_TOO_LONG;
+  const int32_t *sS = useN ? p->rsN : p->rsT;
+  const int32_t *sI = useN ? p->ciN : p->ciT;
+  const dim3 grid(pt->nblocks), block(kBlock);
+  hipLaunchKernelGGL((k_scatter<T>), grid, block, 0, st, pt->d, nrows, sS, sI, useN ? 1 : 0, K, kc,
+                     static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_extract_typed(int32_t N, int32_t K, const int32_t *ids, const void *q, void *gq, hipStream_t st) {
+  const int64_t total = (int64_t)N * K;
+  if (total == 0) return OGS_SUCCESS;
+  hipLaunchKernelGGL((k_extract<T>), dim3(grid_for(total)), dim3(kBlock), 0, st, total, K, ids,
+                     static_cast<const T *>(q), static_cast<T *>(gq));
+  return (int)hipGetLastError();
+}
+
+static bool bad_type(int t) { return t < OGS_FLOAT || t > OGS_INT64; }
+static bool bad_op(int o) { return o < OGS_ADD || o > OGS_MIN; }
+static bool bad_trans(int t) { return t < OGS_SYM || t > OGS_TRANS; }
+
+// ---------------------------------------------------------------------------
+// Plan construction.
+
+static int read_int(const int32_t *d, hipStream_t st, int32_t *h) {
+  hipError_t e = hipMemcpyAsync(h, d, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return (int)e;
+}
+
+static int scan_csr(const int32_t *rs, int nrows, const int32_t *ci, int nnz, int ncols, int32_t *scratch,
+                    hipStream_t st, int32_t *maxRow, int32_t *flags) {
+  hipError_t e = hipMemsetAsync(scratch, 0, 2 * sizeof(int32_t), st);
+  if (e != hipSuccess) return (int)e;
+  const int64_t n = nrows > nnz ? nrows : nnz;
+  hipLaunchKernelGGL(k_scan_csr, dim3(grid_for(n, 4096)), dim3(kBlock), 0, st, rs, nrows, ci, nnz, ncols, scratch);
+  int32_t h[2] = {0, 0};
+  e = hipMemcpyAsync(h, scratch, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return (int)e;
+  *maxRow = h[0];
+  *flags = h[1];
+  return OGS_SUCCESS;
+}
+
+static int build_partition(Plan *p, bool cutOnN, int nrows, int W, int32_t *scratch, hipStream_t st, Partition *out) {
+  const int64_t keys = (int64_t)(cutOnN ? p->nnzN : p->nnzT) + nrows;
+  const int64_t nb = (keys + W - 1) / W;
+  if (nb > INT32_MAX - 1) return OGS_E_ARG;
+  out->W = W;
+  out->nblocks = (int)nb;
+  hipError_t e = hipMalloc(&out->d, sizeof(int4) * (size_t)(nb + 1));
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(k_partition, dim3(grid_for(nb + 1, 1 << 30)), dim3(kBlock), 0, st, p->rsT, p->rsN,
+                     cutOnN ? 1 : 0, nrows, W, (int)nb, out->d);
+  e = hipMemsetAsync(scratch, 0, 3 * sizeof(int32_t), st);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(k_part_max, dim3(grid_for(nb, 2048)), dim3(kBlock), 0, st, out->d, (int)nb, scratch);
+  int32_t h[3];
+  e = hipMemcpyAsync(h, scratch, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return (int)e;
+  out->maxT = h[0];
+  out->maxN = h[1];
+  out->maxRows = h[2];
+  return OGS_SUCCESS;
+}
+
+static void free_plan(Plan *p) {
+  for (int j = 0; j < p->nparts; ++j) {
+    if (p->partT[j].d) (void)hipFree(p->partT[j].d);
+    if (p->partN[j].d && p->partN[j].d != p->partT[j].d) (void)hipFree(p->partN[j].d);
+  }
+  delete p;
+}
+
+}  // namespace ogs_hip
+
+using namespace ogs_hip;
+
+extern "C" {
+
+const char *ogs_hip_error_string(int code) {
+  switch (code) {
+    case OGS_SUCCESS: return "success";
+    case OGS_E_ARG: return "invalid argument";
+    case OGS_E_ROW_TOO_LONG: return "row multiplicity too large";
+    case OGS_E_GATHER_UNDEF: return "gather operation not well-defined";
+    case OGS_E_INVALID_SETUP: return "invalid ogs setup requested";
+    case OGS_E_HALO_POSITIVE: return "halo group without exactly one positive id";
+    case OGS_E_NOT_SUPPORTED: return "type/op combination not supported";
+    case OGS_E_CSR: return "CSR arrays inconsistent with the given sizes";
+    case OGS_E_NO_DEVICE: return "no HIP device";
+    default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
+  }
+}
+
+int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
+                        const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
+                        const int32_t *d_rowStartsT, const int32_t *d_colIdsT, void *stream) {
+  if (!plan) return OGS_E_ARG;
+  *plan = nullptr;
+  if (Ncols < 0 || NrowsN < 0 || NrowsT < 0 || NrowsN > NrowsT) return OGS_E_ARG;
+  if (!d_rowStartsN || !d_rowStartsT) return OGS_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Plan *p = new (std::nothrow) Plan();
+  if (!p) return OGS_E_ARG;
+  p->Ncols = Ncols;
+  p->NrowsN = NrowsN;
+  p->NrowsT = NrowsT;
+  p->rsN = d_rowStartsN;
+  p->ciN = d_colIdsN;
+  p->rsT = d_rowStartsT;
+  p->ciT = d_colIdsT;
+  p->aliased = (d_rowStartsN == d_rowStartsT && d_colIdsN == d_colIdsT);
+  int rc = (int)hipGetDevice(&p->device);
+  if (rc) { delete p; return rc; }
+
+  int32_t t0 = 0, tN = 0, n0 = 0, nN = 0;
+  if ((rc = read_int(d_rowStartsT, st, &t0)) || (rc = read_int(d_rowStartsT + NrowsT, st, &tN)) ||
+      (rc = read_int(d_rowStartsN, st, &n0)) || (rc = read_int(d_rowStartsN + NrowsT, st, &nN))) {
+    delete p;
+    return rc;
+  }
+  if (t0 != 0 || n0 != 0 || tN < 0 || nN < 0) { delete p; return OGS_E_CSR; }
+  p->nnzT = tN;
+  p->nnzN = nN;
+  if ((p->nnzT > 0 && !d_colIdsT) || (p->nnzN > 0 && !d_colIdsN)) { delete p; return OGS_E_ARG; }
+
+  int32_t *scratch = nullptr;
+  if ((rc = (int)hipMalloc(&scratch, 4 * sizeof(int32_t)))) { delete p; return rc; }
+  int32_t flags = 0, fl2 = 0;
+  rc = scan_csr(p->rsT, NrowsT, p->ciT, p->nnzT, Ncols, scratch, st, &p->maxRowT, &flags);
+  if (!rc) {
+    if (p->aliased) { p->maxRowN = p->maxRowT; }
+    else rc = scan_csr(p->rsN, NrowsT, p->ciN, p->nnzN, Ncols, scratch, st, &p->maxRowN, &fl2);
+  }
+  if (!rc && (flags | fl2)) rc = OGS_E_CSR;
+  const int lmax = p->maxRowT > p->maxRowN ? p->maxRowT : p->maxRowN;
+  if (!rc && lmax > kCap) rc = OGS_E_ROW_TOO_LONG;
+  // Partition widths: block nonzeros stay <= kCap >> j.
+  for (int j = 0; !rc && j < kMaxParts && NrowsT > 0; ++j) {
+    const int W = (kCap >> j) - (lmax > 0 ? lmax - 1 : 0);
+    if (W < 1) break;
+    rc = build_partition(p, false, NrowsT, W, scratch, st, &p->partT[j]);
+    if (!rc) {
+      if (p->aliased) p->partN[j] = p->partT[j];
+      else rc = build_partition(p, true, NrowsN, W, scratch, st, &p->partN[j]);
+    }
+    p->nparts = j + 1;
+  }
+  (void)hipFree(scratch);
+  if (rc) { free_plan(p); return rc; }
+  *plan = p;
+  return OGS_SUCCESS;
+}
+
+int ogs_hip_plan_destroy(ogs_hip_plan_t plan) {
+  if (!plan) return OGS_E_ARG;
+  free_plan(plan);
+  return OGS_SUCCESS;
+}
+
+int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
+  if (!p || !info) return OGS_E_ARG;
+  info->NrowsN = p->NrowsN;
+  info->NrowsT = p->NrowsT;
+  info->nnzN = p->nnzN;
+  info->nnzT = p->nnzT;
+  info->maxRowT = p->maxRowT;
+  info->maxRowN = p->maxRowN;
+  info->nblocksK1 = p->nparts ? p->partT[0].nblocks : 0;
+  info->widthK1 = p->nparts ? p->partT[0].W : 0;
+  return OGS_SUCCESS;
+}
+
+int ogs_hip_gather_scatter(ogs_hip_plan_t p, int type, int op, int trans, int32_t K, void *d_v, void *stream) {
+  if (!p || bad_type(type) || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;
+  if (!d_v && p->Ncols > 0) return OGS_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (type) {
+    case OGS_FLOAT: return launch_gs_typed<float>(p, op, trans, K, d_v, st);
+    case OGS_DOUBLE: return launch_gs_typed<double>(p, op, trans, K, d_v, st);
+    case OGS_INT32: return launch_gs_typed<int32_t>(p, op, trans, K, d_v, st);
+    default: return launch_gs_typed<int64_t>(p, op, trans, K, d_v, st);
+  }
+}
+
+int ogs_hip_gather(ogs_hip_plan_t p, int type, int op, int trans, int32_t K, void *d_gv, const void *d_v, void *stream) {
+  if (!p || bad_type(type) || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;
+  if ((!d_v && p->Ncols > 0) || (!d_gv && p->NrowsT > 0)) return OGS_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (type) {
+    case OGS_FLOAT: return launch_gather_typed<float>(p, op, trans, K, d_gv, d_v, st);
+    case OGS_DOUBLE: return launch_gather_typed<double>(p, op, trans, K, d_gv, d_v, st);
+    case OGS_INT32: return launch_gather_typed<int32_t>(p, op, trans, K, d_gv, d_v, st);
+    default: return launch_gather_typed<int64_t>(p, op, trans, K, d_gv, d_v, st);
+  }
+}
+
+int ogs_hip_scatter(ogs_hip_plan_t p, int type, int trans, int32_t K, void *d_v, const void *d_gv, void *stream) {
+  if (!p || bad_type(type) || bad_trans(trans) || K < 1) return OGS_E_ARG;
+  if ((!d_v && p->Ncols > 0) || (!d_gv && p->NrowsT > 0)) return OGS_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (type) {
+    case OGS_FLOAT: return launch_scatter_typed<float>(p, trans, K, d_v, d_gv, st);
+    case OGS_DOUBLE: return launch_scatter_typed<double>(p, trans, K, d_v, d_gv, st);
+    case OGS_INT32: return launch_scatter_typed<int32_t>(p, trans, K, d_v, d_gv, st);
+    default: return launch_scatter_typed<int64_t>(p, trans, K, d_v, d_gv, st);
+  }
+}
+
+int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids, const void *d_q, void *d_gq, void *stream) {
+  if (bad_type(type) || N < 0 || K < 1) return OGS_E_ARG;
+  if (N > 0 && (!d_ids || !d_q || !d_gq)) return OGS_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (type) {
+    case OGS_FLOAT: return launch_extract_typed<float>(N, K, d_ids, d_q, d_gq, st);
+    case OGS_DOUBLE: return launch_extract_typed<double>(N, K, d_ids, d_q, d_gq, st);
+    case OGS_INT32: return launch_extract_typed<int32_t>(N, K, d_ids, d_q, d_gq, st);
+    default: return launch_extract_typed<int64_t>(N, K, d_ids, d_q, d_gq, st);
+  }
+}
+
+}  // extern "C"

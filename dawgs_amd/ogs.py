@@ -1,0 +1,170 @@
+"""Python mirror of libp::ogs at one rank, over the C-ABI of libogs_hip.so.
+
+Mirrors the reference's include/ogs.hpp:216-344 (names, argument order and
+meaning), with device buffers given as torch tensors on the GPU (torch is
+only the allocator and stream provider here) and ids as a numpy int64 array.
+
+    o = ogs_t()
+    o.Setup(N, ids, Unsigned, Auto, unique=False, verbose=False)
+    o.GatherScatter(q, 1, Add, Sym)          # q: torch tensor on cuda
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+# enums: include/ogs.hpp:179-200 (+ bf16 storage for the precision form)
+Float, Double, Int32, Int64, BF16 = 0, 1, 2, 3, 4
+Add, Mul, Max, Min = 0, 1, 2, 3
+Sym, NoTrans, Trans = 0, 1, 2
+Auto, Pairwise, CrystalRouter, AllToAll = 0, 1, 2, 3
+Unsigned, Signed, Halo = 0, 1, 2
+
+
+def type_of(t) -> int:
+    import torch
+    m = {torch.float32: Float, torch.float64: Double, torch.int32: Int32, torch.int64: Int64,
+         torch.bfloat16: BF16}
+    if t.dtype not in m:
+        raise TypeError(f"unsupported dtype {t.dtype}")
+    return m[t.dtype]
+
+
+def _dev_ptr(t) -> int:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device buffer expected (torch tensor on a HIP device)")
+    if not t.is_contiguous():
+        raise ValueError("contiguous buffer expected")
+    return t.data_ptr()
+
+
+def _stream(stream) -> int:
+    if stream is not None:
+        return stream
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Plan:
+    """ogs_hip_plan_t over a caller-owned device CSR pair (operator layer)."""
+
+    def __init__(self, Ncols, NrowsN, NrowsT, rowStartsN, colIdsN, rowStartsT, colIdsT, stream=None):
+        self._keep = (rowStartsN, colIdsN, rowStartsT, colIdsT)
+        h = C.c_void_p()
+        check(lib().ogs_hip_plan_create(C.byref(h), Ncols, NrowsN, NrowsT,
+                                        _dev_ptr(rowStartsN), _dev_ptr(colIdsN) if colIdsN.numel() else None,
+                                        _dev_ptr(rowStartsT), _dev_ptr(colIdsT) if colIdsT.numel() else None,
+                                        _stream(stream)), "ogs_hip_plan_create")
+        self.h = h
+        self.owned = True
+
+    @classmethod
+    def borrowed(cls, handle):
+        p = cls.__new__(cls)
+        p.h = C.c_void_p(handle)
+        p.owned = False
+        p._keep = None
+        return p
+
+    def info(self) -> dict:
+        i = _lib.PlanInfo()
+        check(lib().ogs_hip_plan_info(self.h, C.byref(i)), "ogs_hip_plan_info")
+        return {n: getattr(i, n) for n, _ in i._fields_}
+
+    def GatherScatter(self, v, k, op, trans, stream=None):
+        check(lib().ogs_hip_gather_scatter(self.h, type_of(v), op, trans, k, _dev_ptr(v), _stream(stream)),
+              "ogs_hip_gather_scatter")
+
+    def Gather(self, gv, v, k, op, trans, stream=None):
+        check(lib().ogs_hip_gather(self.h, type_of(v), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
+              "ogs_hip_gather")
+
+    def Scatter(self, v, gv, k, trans, stream=None):
+        check(lib().ogs_hip_scatter(self.h, type_of(v), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
+              "ogs_hip_scatter")
+
+    def __del__(self):
+        try:
+            if self.owned and self.h:
+                lib().ogs_hip_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+def extract(gq, q, ids, k, stream=None):
+    """OKL extract (ogsKernels.okl:167-177): gq[k + n*K] = q[k + ids[n]*K]."""
+    check(lib().ogs_hip_extract(type_of(q), ids.numel(), k, _dev_ptr(ids), _dev_ptr(q), _dev_ptr(gq),
+                                _stream(stream)), "ogs_hip_extract")
+
+
+class ogs_t:
+    """ogs::ogs_t (include/ogs.hpp:216-344) at one rank."""
+
+    def __init__(self):
+        self.h = None
+
+    def Setup(self, N: int, ids: np.ndarray, kind: int = Unsigned, method: int = Auto,
+              unique: bool = False, verbose: bool = False) -> None:
+        """ogs_t::Setup (ogsSetup.cpp:44-190).  `ids` (int64) is rewritten
+        in place with the chosen signs when unique=True (ogsSetup.cpp:139-140)."""
+        if not (isinstance(ids, np.ndarray) and ids.dtype == np.int64 and ids.flags.c_contiguous):
+            raise TypeError("ids must be a contiguous numpy int64 array")
+        if N > len(ids):
+            raise ValueError("N exceeds len(ids)")
+        self.Free()
+        h = C.c_void_p()
+        check(lib().ogs_setup(C.byref(h), N, ids.ctypes.data_as(C.c_void_p), kind, method,
+                              int(unique), int(verbose)), "ogs_setup")
+        self.h = h
+        i = _lib.OgsInfo()
+        check(lib().ogs_get_info(self.h, C.byref(i)), "ogs_get_info")
+        for n, _ in i._fields_:
+            setattr(self, n, getattr(i, n))
+        self.gather_defined = bool(self.gather_defined)
+
+    def Free(self) -> None:
+        if self.h:
+            lib().ogs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Free()
+        except Exception:
+            pass
+
+    @property
+    def plan(self) -> Plan:
+        return Plan.borrowed(lib().ogs_get_plan(self.h))
+
+    def csr(self) -> dict:
+        """Host copies of the local operator's CSR and the 512-nnz row blocks."""
+        a = {
+            "rowStartsN": np.empty(self.NrowsT + 1, np.int32), "colIdsN": np.empty(self.nnzN, np.int32),
+            "rowStartsT": np.empty(self.NrowsT + 1, np.int32), "colIdsT": np.empty(self.nnzT, np.int32),
+            "blockRowStartsN": np.empty(self.NrowBlocksN + 1, np.int32),
+            "blockRowStartsT": np.empty(self.NrowBlocksT + 1, np.int32),
+        }
+        p = lambda x: x.ctypes.data_as(C.c_void_p)
+        check(lib().ogs_get_csr(self.h, p(a["rowStartsN"]), p(a["colIdsN"]), p(a["rowStartsT"]),
+                                p(a["colIdsT"]), p(a["blockRowStartsN"]), p(a["blockRowStartsT"])),
+              "ogs_get_csr")
+        return a
+
+    def GatherScatter(self, v, k: int, op: int, trans: int, stream=None) -> None:
+        check(lib().ogs_gather_scatter(self.h, type_of(v), op, trans, k, _dev_ptr(v), _stream(stream)),
+              "ogs_gather_scatter")
+
+    def Gather(self, gv, v, k: int, op: int, trans: int, stream=None) -> None:
+        check(lib().ogs_gather(self.h, type_of(v), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
+              "ogs_gather")
+
+    def Scatter(self, v, gv, k: int, trans: int, stream=None) -> None:
+        check(lib().ogs_scatter(self.h, type_of(v), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
+              "ogs_scatter")

@@ -1,0 +1,163 @@
+/*
+ * ogs_hip.h -- C-ABI of the MI355X (gfx950) rank-local gather-scatter.
+ *
+ * Two layers, both in libogs_hip.so:
+ *
+ *  1. Operator layer (ogs_hip_*): the device half of ogsOperator_t.  A plan
+ *     wraps one caller-owned CSR pair (N and T lists) on the device and owns
+ *     only the block partitions it derives from them.  It replaces the
+ *     OCCA kernel tables and device overloads of the reference:
+ *       ogsOperator_t::Gather  (deviceMemory)  libs/ogs/ogsOperator.cpp:159-187
+ *       ogsOperator_t::Scatter (deviceMemory)  libs/ogs/ogsOperator.cpp:266-293
+ *       ogsOperator_t::GatherScatter (device)  libs/ogs/ogsOperator.cpp:393-432
+ *       OKL gatherScatter/gather/scatter/extract  libs/ogs/okl/ogsKernels.okl:32-177
+ *       InitializeKernels (JIT tables)          libs/ogs/ogsUtils.cpp:46-123
+ *
+ *  2. Handle layer (ogs_*): the ogs_t object of include/ogs.hpp:216-344 at
+ *     one rank -- Setup from a global numbering, then GatherScatter / Gather
+ *     / Scatter on device buffers.
+ *
+ * Conventions (all entry points):
+ *  - Device buffers are plain device pointers.  `stream` is a hipStream_t
+ *    passed as void* (NULL = the null stream).  Launches are asynchronous.
+ *  - Values for K fields are interleaved, field fastest: v[k + id*K]
+ *    (ogsOperator.cpp:92-94).
+ *  - Return 0 on success; a negative OGS_E_* code for a bad argument or a
+ *    broken contract; a positive hipError_t for a HIP failure.  Nothing
+ *    throws across the boundary.  ogs_hip_error_string() names a code.
+ *  - Plans and handles are immutable after creation; launches from several
+ *    host threads on different streams may share them.
+ */
+#ifndef OGS_HIP_H
+#define OGS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Value types.  0..3 follow ogs::Type (include/ogs.hpp:179); OGS_BF16 is
+ * bfloat16 storage with float accumulation (precision form only). */
+typedef enum { OGS_FLOAT = 0, OGS_DOUBLE = 1, OGS_INT32 = 2, OGS_INT64 = 3, OGS_BF16 = 4 } ogs_type_t;
+/* ogs::Op (include/ogs.hpp:191) */
+typedef enum { OGS_ADD = 0, OGS_MUL = 1, OGS_MAX = 2, OGS_MIN = 3 } ogs_op_t;
+/* ogs::Transpose (include/ogs.hpp:194) */
+typedef enum { OGS_SYM = 0, OGS_NOTRANS = 1, OGS_TRANS = 2 } ogs_trans_t;
+/* ogs::Kind (include/ogs.hpp:200) */
+typedef enum { OGS_UNSIGNED = 0, OGS_SIGNED = 1, OGS_HALO = 2 } ogs_kind_t;
+/* ogs::Method (include/ogs.hpp:197); at one rank every method is the
+ * empty pairwise exchange (ogsAuto.cpp:154-156). */
+typedef enum { OGS_AUTO = 0, OGS_PAIRWISE = 1, OGS_CRYSTAL_ROUTER = 2, OGS_ALLTOALL = 3 } ogs_method_t;
+
+enum {
+  OGS_SUCCESS = 0,
+  OGS_E_ARG = -1,            /* null pointer, bad enum, K < 1, negative size */
+  OGS_E_ROW_TOO_LONG = -2,   /* row longer than the limit (ogsOperator.cpp:460-465) */
+  OGS_E_GATHER_UNDEF = -3,   /* Gather/Scatter on a non-unique ogs (ogsSetup.cpp:861-864) */
+  OGS_E_INVALID_SETUP = -4,  /* unique with Unsigned or Halo kind (ogsSetup.cpp:98-100) */
+  OGS_E_HALO_POSITIVE = -5,  /* Halo group without exactly one positive id (ogsSetup.cpp:286-288) */
+  OGS_E_NOT_SUPPORTED = -6,  /* type/op combination not offered (e.g. bf16 Mul) */
+  OGS_E_CSR = -7,            /* CSR arrays inconsistent with the sizes given */
+  OGS_E_NO_DEVICE = -8       /* no HIP device visible */
+};
+
+/* Longest row the setup accepts (ogsOperator.hpp:123 gatherNodesPerBlock). */
+#define OGS_MAX_ROW 512
+
+const char *ogs_hip_error_string(int code);
+
+/* ------------------------------------------------------------------ */
+/* 1. Operator layer                                                   */
+
+typedef struct ogs_hip_plan_s *ogs_hip_plan_t;
+
+/* Wrap a CSR pair already on the device.  Rows [0,NrowsT) are described by
+ * d_rowStartsT/d_colIdsT (all members) and d_rowStartsN/d_colIdsN (positive
+ * members only); both row-start arrays have NrowsT+1 entries and rows
+ * [NrowsN, NrowsT) of the N list are empty.  Pass the same pointers for N and
+ * T for an Unsigned operator (ogsSetup.cpp:719,728).  Synchronises `stream`.
+ * Replaces ogsOperator_t::setupRowBlocks' device upload (ogsOperator.cpp:510-511)
+ * with the plan's own partitions. */
+int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
+                        const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
+                        const int32_t *d_rowStartsT, const int32_t *d_colIdsT, void *stream);
+int ogs_hip_plan_destroy(ogs_hip_plan_t plan);
+
+/* ogsOperator_t::GatherScatter (ogsOperator.cpp:311-360 semantics, device
+ * path 393-432, but with `op` honoured): in place on d_v (Ncols*K values).
+ * Sym: rows NrowsT, gather T, scatter T; NoTrans: rows NrowsT, gather N,
+ * scatter T; Trans: rows NrowsN, gather T, scatter N. */
+int ogs_hip_gather_scatter(ogs_hip_plan_t plan, int type, int op, int trans, int32_t K,
+                           void *d_v, void *stream);
+/* ogsOperator_t::Gather (ogsOperator.cpp:60-97, device 159-187):
+ * d_gv[k + row*K] = OP over the row's list of d_v.  NoTrans: rows [0,NrowsN)
+ * over the N lists; Trans/Sym: rows [0,NrowsT) over the T lists. */
+int ogs_hip_gather(ogs_hip_plan_t plan, int type, int op, int trans, int32_t K,
+                   void *d_gv, const void *d_v, void *stream);
+/* ogsOperator_t::Scatter (ogsOperator.cpp:206-238, device 266-293):
+ * d_v[k + col*K] = d_gv[k + row*K].  Trans: rows [0,NrowsN) over the N lists;
+ * NoTrans/Sym: rows [0,NrowsT) over the T lists. */
+int ogs_hip_scatter(ogs_hip_plan_t plan, int type, int trans, int32_t K,
+                    void *d_v, const void *d_gv, void *stream);
+/* OKL extract (ogsKernels.okl:167-177; host ogsOperator.cpp:540-559):
+ * d_gq[k + n*K] = d_q[k + d_ids[n]*K] for n < N. */
+int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
+                    const void *d_q, void *d_gq, void *stream);
+
+/* Plan facts, for reporting: block count and width of the K=1 partition
+ * used by GatherScatter, the longest row, and nnz of both lists. */
+typedef struct {
+  int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
+  int32_t nblocksK1, widthK1;
+} ogs_hip_plan_info_t;
+int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
+
+/* ------------------------------------------------------------------ */
+/* 2. Handle layer: ogs_t at one rank                                  */
+
+typedef struct ogs_handle_s *ogs_handle_t;
+
+/* ogs_t::Setup (include/ogs.hpp:221-228; libs/ogs/ogsSetup.cpp:44-190).
+ * ids is a host array of N global ids (0 = ignored, negative = flagged);
+ * with unique=1 the chosen signs are written back into ids, as the
+ * reference does (ogsSetup.cpp:139-140).  The sole positive member of each
+ * group is the member with the smallest local index.  Builds the CSR on the
+ * host, uploads it, and creates the operator plan. */
+int ogs_setup(ogs_handle_t *h, int32_t N, int64_t *ids, int kind, int method,
+              int unique, int verbose);
+int ogs_free(ogs_handle_t h);
+
+typedef struct {
+  int32_t N, Ngather, NlocalT, NhaloT, NlocalP, NhaloP;
+  int64_t NgatherGlobal;
+  int32_t kind, unique, gather_defined;
+  int32_t NrowsN, NrowsT, nnzN, nnzT;
+  int32_t NrowBlocksN, NrowBlocksT; /* the reference's 512-nnz row blocks */
+} ogs_info_t;
+int ogs_get_info(ogs_handle_t h, ogs_info_t *info);
+
+/* Copy the local operator's host CSR out (arrays sized from ogs_get_info:
+ * rowStarts NrowsT+1, colIds nnz, blockRowStarts NrowBlocks+1).  Any
+ * pointer may be NULL to skip it. */
+int ogs_get_csr(ogs_handle_t h, int32_t *rowStartsN, int32_t *colIdsN,
+                int32_t *rowStartsT, int32_t *colIdsT,
+                int32_t *blockRowStartsN, int32_t *blockRowStartsT);
+/* The operator plan of the local operator (owned by the handle). */
+ogs_hip_plan_t ogs_get_plan(ogs_handle_t h);
+
+/* ogs_t::GatherScatter on a device buffer (ogs.cpp:39-123). */
+int ogs_gather_scatter(ogs_handle_t h, int type, int op, int trans, int32_t K,
+                       void *d_v, void *stream);
+/* ogs_t::Gather on device buffers (ogs.cpp:202-298); needs a gather-defined
+ * (unique) setup.  d_gv holds Ngather*K values. */
+int ogs_gather(ogs_handle_t h, int type, int op, int trans, int32_t K,
+               void *d_gv, const void *d_v, void *stream);
+/* ogs_t::Scatter on device buffers (ogs.cpp:392-488). */
+int ogs_scatter(ogs_handle_t h, int type, int trans, int32_t K,
+                void *d_v, const void *d_gv, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

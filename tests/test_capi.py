@@ -1,0 +1,76 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every function
+include/ogs_hip.h declares, and rejects bad arguments before touching the
+device (no compute calls: this container has no GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ogs_hip.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ogs_\w+)\s*\(", src)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for n in ("ogs_hip_plan_create", "ogs_hip_gather_scatter", "ogs_hip_gather", "ogs_hip_scatter",
+              "ogs_hip_extract", "ogs_setup", "ogs_gather_scatter"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from dawgs_amd import _lib
+    L = _lib.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python signature table covers exactly the declared set
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_error_strings():
+    from dawgs_amd import _lib
+    L = _lib.lib()
+    assert L.ogs_hip_error_string(0) == b"success"
+    assert b"too large" in L.ogs_hip_error_string(-2)
+    assert b"not well-defined" in L.ogs_hip_error_string(-3)
+
+
+def test_argument_checks_need_no_device():
+    from dawgs_amd import _lib
+    L = _lib.lib()
+    assert L.ogs_hip_plan_create(None, 0, 0, 0, None, None, None, None, None) == -1
+    h = C.c_void_p()
+    assert L.ogs_hip_plan_create(C.byref(h), 10, 5, 4, None, None, None, None, None) == -1  # NrowsN > NrowsT
+    assert L.ogs_hip_gather_scatter(None, 1, 0, 0, 1, None, None) == -1
+    assert L.ogs_hip_gather(None, 1, 0, 0, 1, None, None, None) == -1
+    assert L.ogs_hip_scatter(None, 1, 0, 1, None, None, None) == -1
+    assert L.ogs_hip_extract(9, 1, 1, None, None, None, None) == -1
+    assert L.ogs_hip_extract(1, 0, 0, None, None, None, None) == -1
+    assert L.ogs_gather_scatter(None, 1, 0, 0, 1, None, None) == -1
+    assert L.ogs_setup(None, 0, None, 0, 0, 0, 0) == -1
+
+
+def test_python_mirror_enums_follow_reference():
+    import dawgs_amd as D
+    # include/ogs.hpp:179-200
+    assert (D.Float, D.Double, D.Int32, D.Int64) == (0, 1, 2, 3)
+    assert (D.Add, D.Mul, D.Max, D.Min) == (0, 1, 2, 3)
+    assert (D.Sym, D.NoTrans, D.Trans) == (0, 1, 2)
+    assert (D.Auto, D.Pairwise, D.CrystalRouter, D.AllToAll) == (0, 1, 2, 3)
+    assert (D.Unsigned, D.Signed, D.Halo) == (0, 1, 2)
+
+
+def test_make_ids_matches_oracle():
+    import numpy as np
+    from dawgs_amd.mesh import make_ids, mesh_counts
+    from oracle import oracle as O
+    for box in [(1, 1, 1, 1), (4, 3, 2, 3), (5, 5, 5, 7)]:
+        a = make_ids(*box)
+        assert np.array_equal(a, O.make_ids(*box))
+        assert a.max() == mesh_counts(*box)["NgatherGlobal"]
