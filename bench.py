@@ -87,6 +87,10 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--p", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, time a plain device copy of the same byte count "
+                         "(the practical streaming ceiling, and a known-size kernel for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -140,6 +144,23 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": B}
+    if args.copy_probe:
+        n = B // 16  # two buffers of B/2 bytes: B bytes moved per copy
+        src = torch.empty(n, dtype=torch.float64, device="cuda").uniform_(generator=gen)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(10):
+            dst.copy_(src)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        cms = ev0.elapsed_time(ev1) / 10
+        roofline["copy_probe"] = {"bytes": 16 * n, "us": round(cms * 1e3, 1),
+                                  "GBps": round(16 * n / (cms * 1e-3) / 1e9, 1)}
+        del src, dst
+        log(f"copy probe: {16 * n} B in {cms * 1e3:.1f} us = {16 * n / (cms * 1e-3) / 1e9:.0f} GB/s")
     log(f"GPU: {ms * 1e3:.1f} us/call, {achieved:.0f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s), "
         f"{ogs.NgatherGlobal / (ms * 1e-3):.4g} DOFS/s")
 

@@ -16,6 +16,16 @@ constexpr int kRowSlots = (kWMax + 1 + kBlock - 1) / kBlock;    // row starts pe
 constexpr int kNumXcd = 8;                                      // MI355X: 8 XCDs x 32 CUs
 constexpr int kMaxParts = 7;                                    // partition widths kCap>>0 .. kCap>>6
 
+// Column-sorted pack word: segment (2 bits) | column offset from the
+// segment's base (19 bits) | CSR position within the block (11 bits).
+constexpr int kPackPosBits = 11;
+constexpr int kPackOffBits = 19;
+constexpr int kPackSegShift = kPackPosBits + kPackOffBits;
+constexpr uint32_t kPackPosMask = (1u << kPackPosBits) - 1;
+constexpr uint32_t kPackOffMask = (1u << kPackOffBits) - 1;
+constexpr int kPackSegs = 4;
+static_assert(kCap <= (1 << kPackPosBits), "CSR positions of a block must fit the pack word");
+
 // One merge-path row partition: nblocks+1 entries {row, rowStartsT[row],
 // rowStartsN[row], 0}; block b covers rows [part[b].x, part[b+1].x).
 struct Partition {
@@ -25,6 +35,16 @@ struct Partition {
   int32_t maxT = 0;     // largest T-list nonzero count of a block
   int32_t maxN = 0;     // largest N-list nonzero count of a block
   int32_t maxRows = 0;  // largest row count of a block (<= W)
+};
+
+// A list's entries in the blocks of one partition, each block's entries
+// sorted by column and packed; base[b] holds the block's segment bases, with
+// base[b].x < 0 for a block left in CSR order (more than kPackSegs segments).
+struct Pack {
+  uint32_t *pk = nullptr;
+  int4 *base = nullptr;
+  int32_t packed = 0;  // blocks packed
+  int32_t plain = 0;   // blocks left in CSR order
 };
 
 }  // namespace ogs_hip
@@ -39,6 +59,10 @@ struct ogs_hip_plan_s {
   int nparts = 0;
   ogs_hip::Partition partT[ogs_hip::kMaxParts];  // cut on T, rows [0, NrowsT)
   ogs_hip::Partition partN[ogs_hip::kMaxParts];  // cut on N, rows [0, NrowsN)
+  // column-sorted packs for the K=1 kernels (partition 0 only)
+  ogs_hip::Pack packTT;  // T list over partT[0]
+  ogs_hip::Pack packNT;  // N list over partT[0]
+  ogs_hip::Pack packNN;  // N list over partN[0]
 };
 
 namespace ogs_hip {

@@ -88,6 +88,7 @@ __device__ __forceinline__ int xcd_block(int wg, int nwg) {
 // Block extent shared by all kernels.  A partition entry is
 // {first row, rowStartsT[row], rowStartsN[row], 0}.
 struct Extent {
+  bool full;   // rows end at the partition block's end (not cut at nrows)
   int r0, nr;  // rows [r0, r0+nr)
   int g0, nG;  // gather list nonzeros [g0, g0+nG)
   int s0, nS;  // scatter list nonzeros [s0, s0+nS)
@@ -104,6 +105,7 @@ __device__ __forceinline__ bool block_extent(const int4 *__restrict__ part, int 
   if (e.r0 >= r1) return false;
   e.nr = r1 - e.r0;
   const bool full = (r1 == p1.x);
+  e.full = full;
   e.g0 = gUseN ? p0.z : p0.y;
   e.nG = (full ? (gUseN ? p1.z : p1.y) : gS[r1]) - e.g0;
   e.s0 = sUseN ? p0.z : p0.y;
@@ -112,87 +114,201 @@ __device__ __forceinline__ bool block_extent(const int4 *__restrict__ part, int 
 }
 
 // ---------------------------------------------------------------------------
-// GatherScatter, K = 1.  SYM: the gather and scatter lists are the same
-// array (Sym, or an Unsigned operator), so colIds stay in registers between
-// the gather and the scatter and the scatter reuses the gather slots.
+// K = 1 kernels.  A block's list entries are visited in the plan's
+// column-sorted order when the list has a pack (consecutive lanes then touch
+// consecutive addresses of v), and in CSR order otherwise.  Either way each
+// gathered value lands in LDS at its CSR position, so the per-row reduction
+// below always runs in CSR order.
+
+// One list of a block as the kernel reads it.  pk/base: the plan's pack
+// (null: plain colIds).  base[b].x < 0 marks a block the pack could not
+// encode; that block reads colIds.
+struct ListView {
+  const int32_t *starts;
+  const int32_t *ids;
+  const uint32_t *pk;
+  const int4 *base;
+};
+
+// Index words of a block's list slots j (list positions tid + j*kBlock < n):
+// pack words when the block is packed, else colIds.  All index loads of a
+// block (these and the row starts) are issued before its first gather.
+// Slots at or past n get word 0, which decodes to a valid column (the first
+// segment base, or column 0), so the gathers can run unpredicated: a load
+// under a branch can make the compiler wait for it at the join.  A
+// block cut short at nrows (full == false) reads in CSR order: its pack sorts
+// the entries of all the block's rows.  Returns whether the block is packed.
+__device__ __forceinline__ bool slot_words(const ListView &L, int b, bool full, int g0, int n,
+                                           uint32_t (&w)[kSlots], int4 &bb) {
+  const int tid = threadIdx.x;
+  bb = make_int4(-1, 0, 0, 0);
+  if (L.pk && full) bb = L.base[b];
+  const bool packed = bb.x >= 0;
+  const uint32_t *src = packed ? L.pk + g0 : reinterpret_cast<const uint32_t *>(L.ids) + g0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int i = tid + j * kBlock;
+    w[j] = i < n ? __builtin_nontemporal_load(src + i) : 0u;
+  }
+  return packed;
+}
+
+// Column of a slot's index word.
+__device__ __forceinline__ int slot_col(uint32_t w, const int4 &bb, bool packed) {
+  if (!packed) return (int)w;
+  const uint32_t sg = w >> kPackSegShift;
+  const int sb = sg == 0 ? bb.x : sg == 1 ? bb.y : sg == 2 ? bb.z : bb.w;
+  return sb + (int)((w >> kPackPosBits) & kPackOffMask);
+}
+
+// LDS (CSR) position of slot j's entry.
+__device__ __forceinline__ int slot_pos(uint32_t w, int j, bool packed) {
+  return packed ? (int)(w & kPackPosMask) : (int)threadIdx.x + j * kBlock;
+}
+
+// Row starts [r0, r0+nr] of a list, loaded into registers.
+__device__ __forceinline__ void row_words(const int32_t *__restrict__ starts, int r0, int nr, int (&v)[kRowSlots]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    v[j] = r <= nr ? __builtin_nontemporal_load(starts + r0 + r) : 0;
+  }
+}
+
+__device__ __forceinline__ void stage_rows(const int (&v)[kRowSlots], int nr, int g0, int32_t *__restrict__ rs) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    if (r <= nr) rs[r] = v[j] - g0;
+  }
+}
+
+// GatherScatter, K = 1.  SYM: gather and scatter lists are the same array
+// (Sym, or an Unsigned operator): the scatter reuses the gather's slots.
 template <typename T, int OP, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_gs_k1(const int4 *__restrict__ part, int nrows,
-                                                  const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
-                                                  const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
-                                                  T *__restrict__ q) {
+__global__ __launch_bounds__(kBlock) void k_gs_k1(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
+                                                  ListView S, int sUseN, T *__restrict__ q) {
   __shared__ T gval[kCap];
   __shared__ T sval[SYM ? 1 : kCap];
   __shared__ int32_t rsG[kWMax + 1];
   __shared__ int32_t rsS[SYM ? 1 : kWMax + 1];
 
   Extent e;
-  if (!block_extent(part, nrows, gS, gUseN, sS, sUseN, e)) return;
+  if (!block_extent(part, nrows, G.starts, gUseN, S.starts, sUseN, e)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
 
-  int c[kSlots];
-  int cs[SYM ? 1 : kSlots];
-  int rg[kRowSlots];
-  int rss[SYM ? 1 : kRowSlots];
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int i = tid + j * kBlock;
-    c[j] = (i < e.nG) ? __builtin_nontemporal_load(gI + e.g0 + i) : 0;
-  }
-  if constexpr (!SYM) {
-#pragma unroll
-    for (int j = 0; j < kSlots; ++j) {
-      const int i = tid + j * kBlock;
-      cs[j] = (i < e.nS) ? __builtin_nontemporal_load(sI + e.s0 + i) : 0;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < kRowSlots; ++j) {
-    const int r = tid + j * kBlock;
-    rg[j] = (r <= e.nr) ? __builtin_nontemporal_load(gS + e.r0 + r) : 0;
-    if constexpr (!SYM) rss[j] = (r <= e.nr) ? __builtin_nontemporal_load(sS + e.r0 + r) : 0;
-  }
+  uint32_t w[kSlots], sw[SYM ? 1 : kSlots];
+  int4 gb, sb;
+  int rg[kRowSlots], rss[SYM ? 1 : kRowSlots];
+  const bool gp = slot_words(G, b, e.full, e.g0, e.nG, w, gb);
+  bool sp = false;
+  if constexpr (!SYM) sp = slot_words(S, b, e.full, e.s0, e.nS, sw, sb);
+  row_words(G.starts, e.r0, e.nr, rg);
+  if constexpr (!SYM) row_words(S.starts, e.r0, e.nr, rss);
   T x[kSlots];
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int i = tid + j * kBlock;
-    if (i < e.nG) x[j] = q[c[j]];
-  }
-#pragma unroll
-  for (int j = 0; j < kRowSlots; ++j) {
-    const int r = tid + j * kBlock;
-    if (r <= e.nr) {
-      rsG[r] = rg[j] - e.g0;
-      if constexpr (!SYM) rsS[r] = rss[j] - e.s0;
-    }
-  }
+  for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, gp)];
+  stage_rows(rg, e.nr, e.g0, rsG);
+  if constexpr (!SYM) stage_rows(rss, e.nr, e.s0, rsS);
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
-    const int i = tid + j * kBlock;
-    if (i < e.nG) gval[i] = x[j];
+    if (tid + j * kBlock < e.nG) gval[slot_pos(w[j], j, gp)] = x[j];
   }
   __syncthreads();
 
   for (int r = tid; r < e.nr; r += kBlock) {
-    const int a = rsG[r], b = rsG[r + 1];
+    const int a = rsG[r], z = rsG[r + 1];
     T acc = Red<T, OP>::init();
-    for (int i = a; i < b; ++i) Red<T, OP>::apply(acc, gval[i]);
+    for (int i = a; i < z; ++i) Red<T, OP>::apply(acc, gval[i]);
     if constexpr (SYM) {
-      for (int i = a; i < b; ++i) gval[i] = acc;
+      for (int i = a; i < z; ++i) gval[i] = acc;
     } else {
-      const int sa = rsS[r], sb = rsS[r + 1];
-      for (int i = sa; i < sb; ++i) sval[i] = acc;
+      const int sa = rsS[r], sz = rsS[r + 1];
+      for (int i = sa; i < sz; ++i) sval[i] = acc;
     }
   }
   __syncthreads();
 
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
-    const int i = tid + j * kBlock;
     if constexpr (SYM) {
-      if (i < e.nG) q[c[j]] = gval[i];
+      if (tid + j * kBlock < e.nG) q[slot_col(w[j], gb, gp)] = gval[slot_pos(w[j], j, gp)];
     } else {
-      if (i < e.nS) q[cs[j]] = sval[i];
+      if (tid + j * kBlock < e.nS) q[slot_col(sw[j], sb, sp)] = sval[slot_pos(sw[j], j, sp)];
     }
+  }
+}
+
+// Gather, K = 1: gq[row] = OP over the row's list.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gather_k1(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
+                                                      const T *__restrict__ q, T *__restrict__ gq) {
+  __shared__ T gval[kCap];
+  __shared__ int32_t rsG[kWMax + 1];
+  Extent e;
+  if (!block_extent(part, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  uint32_t w[kSlots];
+  int4 gb;
+  int rg[kRowSlots];
+  const bool gp = slot_words(G, b, e.full, e.g0, e.nG, w, gb);
+  row_words(G.starts, e.r0, e.nr, rg);
+  T x[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, gp)];
+  stage_rows(rg, e.nr, e.g0, rsG);
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    if (tid + j * kBlock < e.nG) gval[slot_pos(w[j], j, gp)] = x[j];
+  }
+  __syncthreads();
+  for (int r = tid; r < e.nr; r += kBlock) {
+    const int a = rsG[r], z = rsG[r + 1];
+    T acc = Red<T, OP>::init();
+    for (int i = a; i < z; ++i) Red<T, OP>::apply(acc, gval[i]);
+    gq[e.r0 + r] = acc;
+  }
+}
+
+// Scatter, K = 1: q[col] = gq[row] for every member of the row's list.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter_k1(const int4 *__restrict__ part, int nrows, ListView S, int sUseN,
+                                                       const T *__restrict__ gq, T *__restrict__ q) {
+  __shared__ T sval[kCap];
+  __shared__ int32_t rsS[kWMax + 1];
+  Extent e;
+  if (!block_extent(part, nrows, S.starts, sUseN, S.starts, sUseN, e)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  uint32_t w[kSlots];
+  int4 sb;
+  int rss[kRowSlots];
+  const bool sp = slot_words(S, b, e.full, e.s0, e.nS, w, sb);
+  row_words(S.starts, e.r0, e.nr, rss);
+  T x[kRowSlots];
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    x[j] = r < e.nr ? gq[e.r0 + r] : T(0);
+  }
+  stage_rows(rss, e.nr, e.s0, rsS);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int r = tid + j * kBlock;
+    if (r < e.nr) {
+      const int a = rsS[r], z = rsS[r + 1];
+      for (int i = a; i < z; ++i) sval[i] = x[j];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    if (tid + j * kBlock < e.nS) q[slot_col(w[j], sb, sp)] = sval[slot_pos(w[j], j, sp)];
   }
 }
 
@@ -493,6 +609,97 @@ __global__ __launch_bounds__(kBlock) void k_part_max(const int4 *__restrict__ pa
   if (threadIdx.x < 3) atomicMax(&out[threadIdx.x], s[threadIdx.x]);
 }
 
+// Column-sorted pack of one list over one partition: one workgroup per
+// block sorts the block's (column, CSR position) pairs by column (bitonic
+// network in LDS), splits the sorted columns into at most kPackSegs runs
+// whose offsets from the run's first column fit kPackOffBits, and writes the
+// packed words at the block's list positions.  A block needing more runs
+// keeps base.x = -1 and is read in CSR order.
+__global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ part, int nrows,
+                                                       const int32_t *__restrict__ starts,
+                                                       const int32_t *__restrict__ ids, int useN,
+                                                       uint32_t *__restrict__ pk, int4 *__restrict__ base) {
+  __shared__ int32_t key[kCap];
+  __shared__ int16_t val[kCap];
+  __shared__ int32_t segStart[kPackSegs], segBase[kPackSegs];
+  __shared__ int32_t nseg, fail;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int4 p0 = part[b], p1 = part[b + 1];
+  const int r1 = min(p1.x, nrows);
+  int n = 0, g0 = 0;
+  if (p0.x < r1) {
+    g0 = useN ? p0.z : p0.y;
+    n = ((r1 == p1.x) ? (useN ? p1.z : p1.y) : starts[r1]) - g0;
+  }
+  if (n <= 0 || n > kCap) {
+    if (tid == 0) base[b] = make_int4(-1, 0, 0, 0);
+    return;
+  }
+  int n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  for (int t = tid; t < n2; t += kBlock) {
+    key[t] = t < n ? ids[g0 + t] : INT_MAX;
+    val[t] = (int16_t)t;
+  }
+  __syncthreads();
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = tid; t < n2; t += kBlock) {
+        const int u = t ^ j;
+        if (u > t) {
+          const bool up = (t & k) == 0;
+          const int a = key[t], c = key[u];
+          if ((a > c) == up) {
+            key[t] = c;
+            key[u] = a;
+            const int16_t va = val[t];
+            val[t] = val[u];
+            val[u] = va;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    int ns = 0, bad = 0;
+    for (int t = 0; t < n; ++t) {
+      if (ns == 0 || (uint32_t)(key[t] - segBase[ns - 1]) > kPackOffMask) {
+        if (ns == kPackSegs) { bad = 1; break; }
+        segStart[ns] = t;
+        segBase[ns] = key[t];
+        ++ns;
+      }
+    }
+    nseg = ns;
+    fail = bad;
+  }
+  __syncthreads();
+  if (fail) {
+    if (tid == 0) base[b] = make_int4(-1, 0, 0, 0);
+    return;
+  }
+  for (int t = tid; t < n; t += kBlock) {
+    int sg = 0;
+    for (int s = 1; s < nseg; ++s) sg += (t >= segStart[s]);
+    pk[g0 + t] = ((uint32_t)sg << kPackSegShift) | ((uint32_t)(key[t] - segBase[sg]) << kPackPosBits) |
+                 (uint32_t)val[t];
+  }
+  if (tid == 0) {
+    int bs[kPackSegs] = {0, 0, 0, 0};
+    for (int s = 0; s < nseg; ++s) bs[s] = segBase[s];
+    base[b] = make_int4(bs[0], bs[1], bs[2], bs[3]);
+  }
+}
+
+// Count of packed blocks (base.x >= 0) of a pack.
+__global__ __launch_bounds__(kBlock) void k_count_packed(const int4 *__restrict__ base, int nblocks,
+                                                         int32_t *__restrict__ out) {
+  int c = 0;
+  for (int b = blockIdx.x * kBlock + threadIdx.x; b < nblocks; b += gridDim.x * kBlock) c += base[b].x >= 0;
+  atomicAdd(out, c);
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers.
 
@@ -522,20 +729,41 @@ static const Partition *choose(const Partition *parts, int nparts, int K, MaxOf 
   return best;
 }
 
+// The pack of a list over a partition, when the launch uses partition 0.
+static ListView list_view(const Plan *p, bool listN, const Partition *pt, bool partIsN) {
+  ListView L;
+  L.starts = listN ? p->rsN : p->rsT;
+  L.ids = listN ? p->ciN : p->ciT;
+  L.pk = nullptr;
+  L.base = nullptr;
+  const Pack *pk = nullptr;
+  if (pt == (partIsN ? &p->partN[0] : &p->partT[0])) {
+    if (partIsN) pk = listN ? &p->packNN : nullptr;
+    else pk = listN ? &p->packNT : &p->packTT;
+  }
+  if (pk && pk->pk) {
+    L.pk = pk->pk;
+    L.base = pk->base;
+  }
+  return L;
+}
+
 template <typename T>
 static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:321-339
   int nrows;
-  const int32_t *gS, *gI, *sS, *sI;
-  int gN, sN;
+  bool gListN, sListN;
   if (trans == OGS_TRANS) {
-    nrows = p->NrowsN; gS = p->rsT; gI = p->ciT; gN = 0; sS = p->rsN; sI = p->ciN; sN = 1;
+    nrows = p->NrowsN; gListN = false; sListN = true;
   } else if (trans == OGS_SYM) {
-    nrows = p->NrowsT; gS = p->rsT; gI = p->ciT; gN = 0; sS = p->rsT; sI = p->ciT; sN = 0;
+    nrows = p->NrowsT; gListN = false; sListN = false;
   } else {
-    nrows = p->NrowsT; gS = p->rsN; gI = p->ciN; gN = 1; sS = p->rsT; sI = p->ciT; sN = 0;
+    nrows = p->NrowsT; gListN = true; sListN = false;
   }
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  const int32_t *gS = gListN ? p->rsN : p->rsT, *gI = gListN ? p->ciN : p->ciT;
+  const int32_t *sS = sListN ? p->rsN : p->rsT, *sI = sListN ? p->ciN : p->ciT;
+  const int gN = gListN ? 1 : 0, sN = sListN ? 1 : 0;
   const bool sym = (gI == sI && gS == sS);
   int kc = 0;
   const Partition *pt = choose(p->partT, p->nparts, K,
@@ -543,10 +771,11 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   if (!pt) return OGS_E_ROW_TOO_LONG;
   const dim3 grid(pt->nblocks), block(kBlock);
   T *v = static_cast<T *>(d_v);
+  const ListView G = list_view(p, gListN, pt, false), S = list_view(p, sListN, pt, false);
 #define OGS_GS_LAUNCH(OPV)                                                                                      \
   if (K == 1) {                                                                                                 \
-    if (sym) hipLaunchKernelGGL((k_gs_k1<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, v); \
-    else hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, v);   \
+    if (sym) hipLaunchKernelGGL((k_gs_k1<T, OPV, true>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);    \
+    else hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);       \
   } else {                                                                                                      \
     if (sym) hipLaunchKernelGGL((k_gs_kv<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v); \
     else hipLaunchKernelGGL((k_gs_kv<T, OPV, false>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v);   \
@@ -568,7 +797,8 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
   const bool useN = (trans == OGS_NOTRANS);
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
-  const Partition *parts = (useN && !p->aliased) ? p->partN : p->partT;
+  const bool partN = useN && !p->aliased;
+  const Partition *parts = partN ? p->partN : p->partT;
   int kc = 0;
   const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
   if (!pt) return OGS_E_ROW_TOO_LONG;
@@ -578,13 +808,18 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
   const dim3 grid(pt->nblocks), block(kBlock);
   const T *v = static_cast<const T *>(d_v);
   T *gv = static_cast<T *>(d_gv);
+  const ListView G = list_view(p, useN && !p->aliased, pt, partN);
+#define OGS_G_LAUNCH(OPV)                                                                                          \
+  if (K == 1) hipLaunchKernelGGL((k_gather_k1<T, OPV>), grid, block, 0, st, pt->d, nrows, G, gN, v, gv);          \
+  else hipLaunchKernelGGL((k_gather<T, OPV>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv);
   switch (op) {
-    case OGS_ADD: hipLaunchKernelGGL((k_gather<T, OGS_ADD>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
-    case OGS_MUL: hipLaunchKernelGGL((k_gather<T, OGS_MUL>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
-    case OGS_MAX: hipLaunchKernelGGL((k_gather<T, OGS_MAX>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
-    case OGS_MIN: hipLaunchKernelGGL((k_gather<T, OGS_MIN>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv); break;
+    case OGS_ADD: OGS_G_LAUNCH(OGS_ADD) break;
+    case OGS_MUL: OGS_G_LAUNCH(OGS_MUL) break;
+    case OGS_MAX: OGS_G_LAUNCH(OGS_MAX) break;
+    case OGS_MIN: OGS_G_LAUNCH(OGS_MIN) break;
     default: return OGS_E_ARG;
   }
+#undef OGS_G_LAUNCH
   return (int)hipGetLastError();
 }
 
@@ -594,15 +829,22 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const bool useN = (trans == OGS_TRANS);
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
-  const Partition *parts = (useN && !p->aliased) ? p->partN : p->partT;
+  const bool partN = useN && !p->aliased;
+  const Partition *parts = partN ? p->partN : p->partT;
   int kc = 0;
   const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
   if (!pt) return OGS_E_ROW_TOO_LONG;
   const int32_t *sS = useN ? p->rsN : p->rsT;
   const int32_t *sI = useN ? p->ciN : p->ciT;
   const dim3 grid(pt->nblocks), block(kBlock);
-  hipLaunchKernelGGL((k_scatter<T>), grid, block, 0, st, pt->d, nrows, sS, sI, useN ? 1 : 0, K, kc,
-                     static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+  if (K == 1) {
+    const ListView S = list_view(p, useN && !p->aliased, pt, partN);
+    hipLaunchKernelGGL((k_scatter_k1<T>), grid, block, 0, st, pt->d, nrows, S, useN ? 1 : 0,
+                       static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+  } else {
+    hipLaunchKernelGGL((k_scatter<T>), grid, block, 0, st, pt->d, nrows, sS, sI, useN ? 1 : 0, K, kc,
+                       static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+  }
   return (int)hipGetLastError();
 }
 
@@ -666,7 +908,39 @@ static int build_partition(Plan *p, bool cutOnN, int nrows, int W, int32_t *scra
   return OGS_SUCCESS;
 }
 
+static int build_pack(const Partition &pt, int nrows, const int32_t *starts, const int32_t *ids, bool useN,
+                      int nnz, int32_t *scratch, hipStream_t st, Pack *out) {
+  if (pt.nblocks <= 0 || nnz <= 0) return OGS_SUCCESS;
+  hipError_t e = hipMalloc(&out->pk, sizeof(uint32_t) * (size_t)nnz);
+  if (e == hipSuccess) e = hipMalloc(&out->base, sizeof(int4) * (size_t)pt.nblocks);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(k_build_pack, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, nrows, starts, ids, useN ? 1 : 0,
+                     out->pk, out->base);
+  e = hipMemsetAsync(scratch, 0, sizeof(int32_t), st);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(k_count_packed, dim3(grid_for(pt.nblocks, 1024)), dim3(kBlock), 0, st, out->base, pt.nblocks,
+                     scratch);
+  int32_t h = 0;
+  e = hipMemcpyAsync(&h, scratch, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return (int)e;
+  out->packed = h;
+  out->plain = pt.nblocks - h;
+  return OGS_SUCCESS;
+}
+
+static void free_pack(Pack &k) {
+  if (k.pk) (void)hipFree(k.pk);
+  if (k.base) (void)hipFree(k.base);
+  k = Pack();
+}
+
 static void free_plan(Plan *p) {
+  const bool nt = p->packNT.pk && p->packNT.pk != p->packTT.pk;
+  const bool nn = p->packNN.pk && p->packNN.pk != p->packTT.pk;
+  if (nt) free_pack(p->packNT);
+  if (nn) free_pack(p->packNN);
+  free_pack(p->packTT);
   for (int j = 0; j < p->nparts; ++j) {
     if (p->partT[j].d) (void)hipFree(p->partT[j].d);
     if (p->partN[j].d && p->partN[j].d != p->partT[j].d) (void)hipFree(p->partN[j].d);
@@ -749,6 +1023,17 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     }
     p->nparts = j + 1;
   }
+  // column-sorted packs of the K=1 partitions
+  if (!rc && p->nparts > 0) {
+    rc = build_pack(p->partT[0], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packTT);
+    if (!rc && p->aliased) {
+      p->packNT = p->packTT;
+      p->packNN = p->packTT;
+    } else if (!rc) {
+      rc = build_pack(p->partT[0], NrowsT, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNT);
+      if (!rc) rc = build_pack(p->partN[0], NrowsN, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNN);
+    }
+  }
   (void)hipFree(scratch);
   if (rc) { free_plan(p); return rc; }
   *plan = p;
@@ -771,6 +1056,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->maxRowN = p->maxRowN;
   info->nblocksK1 = p->nparts ? p->partT[0].nblocks : 0;
   info->widthK1 = p->nparts ? p->partT[0].W : 0;
+  info->packedK1 = p->packTT.packed;
+  info->plainK1 = p->packTT.plain;
   return OGS_SUCCESS;
 }
 

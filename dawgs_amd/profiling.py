@@ -1,0 +1,96 @@
+"""Summaries of rocprofv3 output for the gather-scatter kernels.
+
+    python -m dawgs_amd.profiling stats  <dir-with-*_kernel_stats.csv>
+    python -m dawgs_amd.profiling traffic <FETCH_SIZE run dir> <WRITE_SIZE run dir> \
+        --kernel k_gs_k1 --copy-kernel copy --copy-bytes B [--json out.json]
+
+`traffic` turns two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs
+into HBM bytes per launch of a kernel.  Each counter is scaled by what it
+reads for a plain copy kernel of known size in the same run (on MI355X,
+FETCH_SIZE counts half of a copy's read bytes; WRITE_SIZE counts them all).
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import sys
+
+
+def _rows(d: str, suffix: str):
+    files = glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    out = []
+    for f in files:
+        with open(f) as fh:
+            out.extend(csv.DictReader(fh))
+    return out
+
+
+def kernel_stats(d: str):
+    rows = _rows(d, "kernel_stats.csv")
+    return [{"name": r["Name"], "calls": int(r["Calls"]), "avg_ns": float(r["AverageNs"]),
+             "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]), "pct": float(r["Percentage"])}
+            for r in rows]
+
+
+def counter_means(d: str, counter: str):
+    """Mean counter value (KiB as rocprofv3 reports it) per kernel name."""
+    acc = {}
+    for r in _rows(d, "counter_collection.csv"):
+        if r["Counter_Name"] != counter:
+            continue
+        acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def _pick(means: dict, pattern: str, exclude: str = "rocclr") -> float:
+    hits = [v for k, v in means.items() if pattern in k and exclude not in k]
+    if len(hits) != 1:
+        raise KeyError(f"{len(hits)} kernels match {pattern!r}: {[k for k in means if pattern in k]}")
+    return hits[0]
+
+
+def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_kernel: str, copy_bytes: int) -> dict:
+    f = counter_means(fetch_dir, "FETCH_SIZE")
+    w = counter_means(write_dir, "WRITE_SIZE")
+    half = copy_bytes / 2  # a copy reads half and writes half of the bytes it moves
+    f_scale = half / (_pick(f, copy_kernel) * 1024)
+    w_scale = half / (_pick(w, copy_kernel) * 1024)
+    rd = _pick(f, kernel) * 1024 * f_scale
+    wr = _pick(w, kernel) * 1024 * w_scale
+    return {"kernel": kernel, "fetch_bytes": round(rd), "write_bytes": round(wr),
+            "hbm_bytes_per_launch": round(rd + wr), "fetch_scale": round(f_scale, 4),
+            "write_scale": round(w_scale, 4), "copy_kernel": copy_kernel, "copy_bytes": copy_bytes}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--copy-kernel", required=True)
+    t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--json")
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        for k in sorted(kernel_stats(a.dir), key=lambda r: -r["pct"]):
+            print(f"{k['avg_ns'] / 1e3:10.1f} us avg  {k['calls']:5d} calls  {k['pct']:6.2f}%  {k['name'][:110]}")
+    else:
+        r = traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_kernel, a.copy_bytes)
+        print(json.dumps(r, indent=1))
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump(r, fh, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

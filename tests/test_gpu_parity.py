@@ -384,6 +384,42 @@ def test_hand_built_operator_long_rows(torch, D):
             assert same(to_host(torch, d), ws), (K, trans)
 
 
+def test_unsortable_blocks_fall_back_to_csr_order(torch, D):
+    """Columns scattered over 4M entries: a block's sorted columns need more
+    than 4 runs of 2^19, so the plan leaves such blocks in CSR order.  Both
+    kinds of block must give the oracle's result."""
+    rng = np.random.default_rng(9)
+    Ncols = 4_000_000
+    lens = rng.integers(1, 9, 40_000)
+    lens[:2000] = 1  # short rows over a narrow column range: packable blocks
+    nnz = int(lens.sum())
+    narrow = np.sort(rng.choice(np.arange(Ncols - 50_000, Ncols), 2000, replace=False))
+    wide = rng.choice(Ncols - 50_000, nnz - 2000, replace=False)
+    cols = np.concatenate([narrow, wide]).astype(np.int32)
+    rsT = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ciT = np.concatenate([np.sort(cols[rsT[r]:rsT[r + 1]]) for r in range(len(lens))]).astype(np.int32)
+    assert len(np.unique(ciT)) == nnz
+    ref = O.OracleCsr(rsT, ciT, Ncols=Ncols)
+    d_rs, d_ci = to_dev(torch, rsT), to_dev(torch, ciT)
+    plan = D.Plan(Ncols, len(lens), len(lens), d_rs, d_ci, d_rs, d_ci)
+    info = plan.info()
+    assert info["plainK1"] > 0 and info["packedK1"] > 0, info
+    v = rng.uniform(-1, 1, Ncols)
+    for op in OPS:
+        want = ref.gather_scatter(v.copy(), 1, op, O.SYM)
+        d = to_dev(torch, v)
+        plan.GatherScatter(d, 1, op, O.SYM)
+        assert same(to_host(torch, d), want), op
+    wg = ref.gather(v, 1, O.ADD, O.TRANS)
+    gv = torch.zeros(len(lens), dtype=torch.float64, device="cuda")
+    plan.Gather(gv, to_dev(torch, v), 1, O.ADD, O.TRANS)
+    assert same(to_host(torch, gv), wg)
+    ws = ref.scatter(v.copy(), wg, 1, O.NOTRANS)
+    d = to_dev(torch, v)
+    plan.Scatter(d, gv, 1, O.NOTRANS)
+    assert same(to_host(torch, d), ws)
+
+
 @pytest.mark.parametrize("ty", list(TYPES))
 @pytest.mark.parametrize("K", [1, 3])
 def test_extract(torch, D, ty, K):
