@@ -17,12 +17,16 @@ constexpr int kNumXcd = 8;                                      // MI355X: 8 XCD
 constexpr int kMaxParts = 7;                                    // partition widths kCap>>0 .. kCap>>6
 
 // Column-sorted pack word: segment (2 bits) | column offset from the
-// segment's base (19 bits) | CSR position within the block (11 bits).
+// segment's base (18 bits) | row-head bit (the entry is the first of its row
+// in CSR order) | CSR position within the block (11 bits).
 constexpr int kPackPosBits = 11;
-constexpr int kPackOffBits = 19;
-constexpr int kPackSegShift = kPackPosBits + kPackOffBits;
+constexpr int kPackHeadBit = kPackPosBits;
+constexpr int kPackOffShift = kPackPosBits + 1;
+constexpr int kPackOffBits = 18;
+constexpr int kPackSegShift = kPackOffShift + kPackOffBits;
 constexpr uint32_t kPackPosMask = (1u << kPackPosBits) - 1;
 constexpr uint32_t kPackOffMask = (1u << kPackOffBits) - 1;
+static_assert(kPackSegShift == 30, "pack word is 2 + 18 + 1 + 11 bits");
 constexpr int kPackSegs = 4;
 static_assert(kCap <= (1 << kPackPosBits), "CSR positions of a block must fit the pack word");
 

@@ -158,7 +158,7 @@ __device__ __forceinline__ int slot_col(uint32_t w, const int4 &bb, bool packed)
   if (!packed) return (int)w;
   const uint32_t sg = w >> kPackSegShift;
   const int sb = sg == 0 ? bb.x : sg == 1 ? bb.y : sg == 2 ? bb.z : bb.w;
-  return sb + (int)((w >> kPackPosBits) & kPackOffMask);
+  return sb + (int)((w >> kPackOffShift) & kPackOffMask);
 }
 
 // LDS (CSR) position of slot j's entry.
@@ -182,6 +182,82 @@ __device__ __forceinline__ void stage_rows(const int (&v)[kRowSlots], int nr, in
   for (int j = 0; j < kRowSlots; ++j) {
     const int r = tid + j * kBlock;
     if (r <= nr) rs[r] = v[j] - g0;
+  }
+}
+
+// GatherScatter, K = 1, gather list == scatter list (Sym, or an Unsigned
+// operator).  No row starts are read for a packed block: the pack words
+// carry a head mark at each row's first CSR position, and each row is reduced
+// from its head to the next head, in CSR order.  A block left in CSR order
+// marks its heads from the row starts.  Empty rows have nothing to gather or
+// write, so they need no mark.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_k1(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
+                                                      T *__restrict__ q) {
+  __shared__ T val[kCap];
+  __shared__ uint8_t head[kCap];
+
+  Extent e;
+  if (!block_extent(part, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
+  const int n = e.nG;
+  if (n == 0) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+
+  uint32_t w[kSlots];
+  int4 gb;
+  int rg[kRowSlots];
+  const bool packed = slot_words(G, b, e.full, e.g0, n, w, gb);
+  if (!packed) row_words(G.starts, e.r0, e.nr, rg);
+  T x[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, packed)];
+  if (packed) {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      if (tid + j * kBlock < n) {
+        const int p = (int)(w[j] & kPackPosMask);
+        val[p] = x[j];
+        head[p] = (uint8_t)((w[j] >> kPackHeadBit) & 1u);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int i = tid + j * kBlock;
+      if (i < n) {
+        val[i] = x[j];
+        head[i] = 0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRowSlots; ++j) {
+      const int r = tid + j * kBlock;
+      const int a = rg[j] - e.g0;
+      if (r < e.nr && a < n) head[a] = 1;  // an empty row marks the next row's head again
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int i = tid + j * kBlock;
+    if (i < n && head[i]) {
+      T acc = Red<T, OP>::init();
+      int k = i;
+      do {
+        Red<T, OP>::apply(acc, val[k]);
+        ++k;
+      } while (k < n && !head[k]);
+      for (int m = i; m < k; ++m) val[m] = acc;
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    if (tid + j * kBlock < n) q[slot_col(w[j], gb, packed)] = val[slot_pos(w[j], j, packed)];
   }
 }
 
@@ -209,8 +285,10 @@ __global__ __launch_bounds__(kBlock) void k_gs_k1(const int4 *__restrict__ part,
   row_words(G.starts, e.r0, e.nr, rg);
   if constexpr (!SYM) row_words(S.starts, e.r0, e.nr, rss);
   T x[kSlots];
+  if (e.nG > 0) {  // uniform: an operator may have rows but no columns
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, gp)];
+    for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, gp)];
+  }
   stage_rows(rg, e.nr, e.g0, rsG);
   if constexpr (!SYM) stage_rows(rss, e.nr, e.s0, rsS);
 #pragma unroll
@@ -258,8 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_k1(const int4 *__restrict__ p
   const bool gp = slot_words(G, b, e.full, e.g0, e.nG, w, gb);
   row_words(G.starts, e.r0, e.nr, rg);
   T x[kSlots];
+  if (e.nG > 0) {  // uniform: an operator may have rows but no columns
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, gp)];
+    for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, gp)];
+  }
   stage_rows(rg, e.nr, e.g0, rsG);
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
@@ -613,7 +693,8 @@ __global__ __launch_bounds__(kBlock) void k_part_max(const int4 *__restrict__ pa
 // block sorts the block's (column, CSR position) pairs by column (bitonic
 // network in LDS), splits the sorted columns into at most kPackSegs runs
 // whose offsets from the run's first column fit kPackOffBits, and writes the
-// packed words at the block's list positions.  A block needing more runs
+// packed words at the block's list positions, each marked with whether its
+// entry is the first of its row.  A block needing more runs
 // keeps base.x = -1 and is read in CSR order.
 __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ part, int nrows,
                                                        const int32_t *__restrict__ starts,
@@ -621,6 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ 
                                                        uint32_t *__restrict__ pk, int4 *__restrict__ base) {
   __shared__ int32_t key[kCap];
   __shared__ int16_t val[kCap];
+  __shared__ uint8_t head[kCap];
   __shared__ int32_t segStart[kPackSegs], segBase[kPackSegs];
   __shared__ int32_t nseg, fail;
   const int b = blockIdx.x, tid = threadIdx.x;
@@ -640,6 +722,12 @@ __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ 
   for (int t = tid; t < n2; t += kBlock) {
     key[t] = t < n ? ids[g0 + t] : INT_MAX;
     val[t] = (int16_t)t;
+    head[t] = 0;
+  }
+  __syncthreads();
+  for (int r = p0.x + tid; r < r1; r += kBlock) {  // first entry of each non-empty row
+    const int a = starts[r] - g0;
+    if (starts[r + 1] - g0 > a) head[a] = 1;
   }
   __syncthreads();
   for (int k = 2; k <= n2; k <<= 1) {
@@ -682,8 +770,8 @@ __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ 
   for (int t = tid; t < n; t += kBlock) {
     int sg = 0;
     for (int s = 1; s < nseg; ++s) sg += (t >= segStart[s]);
-    pk[g0 + t] = ((uint32_t)sg << kPackSegShift) | ((uint32_t)(key[t] - segBase[sg]) << kPackPosBits) |
-                 (uint32_t)val[t];
+    pk[g0 + t] = ((uint32_t)sg << kPackSegShift) | ((uint32_t)(key[t] - segBase[sg]) << kPackOffShift) |
+                 ((uint32_t)head[val[t]] << kPackHeadBit) | (uint32_t)val[t];
   }
   if (tid == 0) {
     int bs[kPackSegs] = {0, 0, 0, 0};
@@ -774,7 +862,7 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   const ListView G = list_view(p, gListN, pt, false), S = list_view(p, sListN, pt, false);
 #define OGS_GS_LAUNCH(OPV)                                                                                      \
   if (K == 1) {                                                                                                 \
-    if (sym) hipLaunchKernelGGL((k_gs_k1<T, OPV, true>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);    \
+    if (sym) hipLaunchKernelGGL((k_gs_sym_k1<T, OPV>), grid, block, 0, st, pt->d, nrows, G, gN, v);              \
     else hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);       \
   } else {                                                                                                      \
     if (sym) hipLaunchKernelGGL((k_gs_kv<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v); \
