@@ -37,6 +37,18 @@ def kernel_stats(d: str):
             for r in rows]
 
 
+def trace_tail_mean(d: str, kernel: str, last: int) -> dict:
+    """Mean duration of the last `last` dispatches of a kernel in a
+    --kernel-trace run (the timed region of a benchmark that launches its
+    timed steps last)."""
+    rows = [r for r in _rows(d, "kernel_trace.csv") if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows]
+    tail = durs[-last:] if last else durs
+    return {"dispatches": len(durs), "tail": len(tail), "tail_mean_ns": sum(tail) / max(len(tail), 1),
+            "all_mean_ns": sum(durs) / max(len(durs), 1)}
+
+
 def counter_means(d: str, counter: str):
     """Mean counter value (KiB as rocprofv3 reports it) per kernel name."""
     acc = {}
@@ -72,6 +84,10 @@ def main(argv=None) -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("dir")
+    tl = sub.add_parser("tail")
+    tl.add_argument("dir")
+    tl.add_argument("--kernel", required=True)
+    tl.add_argument("--last", type=int, required=True)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -83,6 +99,8 @@ def main(argv=None) -> int:
     if a.cmd == "stats":
         for k in sorted(kernel_stats(a.dir), key=lambda r: -r["pct"]):
             print(f"{k['avg_ns'] / 1e3:10.1f} us avg  {k['calls']:5d} calls  {k['pct']:6.2f}%  {k['name'][:110]}")
+    elif a.cmd == "tail":
+        print(json.dumps(trace_tail_mean(a.dir, a.kernel, a.last), indent=1))
     else:
         r = traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_kernel, a.copy_bytes)
         print(json.dumps(r, indent=1))
