@@ -79,6 +79,72 @@ def cpu_baseline(csr, N, NgatherGlobal, budget_s=12.0, max_calls=20):
     }
 
 
+def host_lines(res, log) -> dict:
+    """Host rows of PerformanceTest: the reference's host loops (oracle,
+    OpenMP) over the same CSRs as the device rows."""
+    from dawgs_amd.perf import line
+    from oracle import oracle as O
+    ogs, sogs = res["_ogs"], res["_sogs"]
+    N, Ng, p = res["N"], res["NgatherGlobal"], res["p"]
+    c = ogs.csr()
+    u = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=N)
+    c = sogs.csr()
+    sg = O.OracleCsr(c["rowStartsT"], c["colIdsT"], c["rowStartsN"], c["colIdsN"], sogs.NrowsN, N)
+    v = np.random.default_rng(2).uniform(-1, 1, N)
+    out = {}
+
+    def timed(fn, n=10):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        return (time.perf_counter() - t0) / n
+
+    t = timed(lambda: u.gather_scatter(v, 1, O.ADD, O.SYM))
+    log(line("Host  ", "GatherScatter", Ng, N, p, t, res["gs"]["bytes"]))
+    out["gs"] = t
+    gv = sg.gather(v, 1, O.ADD, O.TRANS)
+    t = timed(lambda: sg.gather(v, 1, O.ADD, O.TRANS))
+    log(line("Host  ", "Gather", Ng, N, p, t, res["gather"]["bytes"]))
+    out["gather"] = t
+    t = timed(lambda: sg.scatter(v, gv, 1, O.NOTRANS))
+    log(line("Host  ", "Scatter", Ng, N, p, t, res["scatter"]["bytes"]))
+    out["scatter"] = t
+    return out
+
+
+def run_sweep(args) -> int:
+    """Degree sweep p = 1..7 at ~56 M local DOFs per run."""
+    import gc
+
+    from dawgs_amd.perf import SWEEP_NX, performance_test
+    from oracle import oracle as O
+    rows = []
+    for p in [int(x) for x in args.sweep.split(",")]:
+        nx = SWEEP_NX[p]
+        log(f" ---- degree {p}: {nx}^3 elements ----")
+        res = performance_test(nx, nx, nx, p, warmup=args.warmup, steps=args.steps, log=log)
+        row = {k: v for k, v in res.items() if not k.startswith("_")}
+        if not args.no_cpu:
+            row["host_s"] = host_lines(res, log)
+        rows.append(row)
+        del res
+        gc.collect()
+    out = {"metric": "ogs degree sweep (Device/Host GatherScatter, Gather, Scatter; fp64 Add; ~56M local DOFs)",
+           "unit": "DOFS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+           "data": "synthetic", "host_threads": O.num_threads() if not args.no_cpu else None,
+           "host_cpu": host_cpu() if not args.no_cpu else None,
+           "rows": [{"p": r["p"], "nx": r["nx"], "local_dofs": r["N"], "global_dofs": r["NgatherGlobal"],
+                     "gs_DOFSps": r["NgatherGlobal"] / r["gs"]["s"], "gs_GBps": r["gs"]["GBps"],
+                     "gs_frac": r["gs"]["GBps"] / HBM_PEAK_GBS,
+                     "gather_DOFSps": r["NgatherGlobal"] / r["gather"]["s"], "gather_GBps": r["gather"]["GBps"],
+                     "scatter_DOFSps": r["NgatherGlobal"] / r["scatter"]["s"], "scatter_GBps": r["scatter"]["GBps"],
+                     "host_gs_DOFSps": (r["NgatherGlobal"] / r["host_s"]["gs"]) if "host_s" in r else None}
+                    for r in rows]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +154,10 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--p", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--sweep", nargs="?", const="1,2,3,4,5,6,7", default=None, metavar="DEGREES",
+                    help="degree sweep at ~56 M local DOFs (48^3 at p=7): Device (and, unless --no-cpu, "
+                         "Host) GatherScatter/Gather/Scatter lines in the reference's format; one JSON "
+                         "summary line on stdout")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, time a plain device copy of the same byte count "
                          "(the practical streaming ceiling, and a known-size kernel for scaling "
@@ -108,6 +178,8 @@ def main() -> int:
         log("no HIP device visible")
         return 1
     torch.cuda.set_device(0)
+    if args.sweep:
+        return run_sweep(args)
     nx, p = args.nx, args.p
 
     t0 = time.perf_counter()
