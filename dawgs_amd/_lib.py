@@ -19,7 +19,16 @@ _I32P = C.POINTER(C.c_int32)
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
-                 "packedK1", "plainK1")]
+                 "packedK1", "plainK1", "Ncols")]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("K", C.c_int32), ("d_v", C.c_void_p)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("items", C.c_int32), ("grid", C.c_int32), ("narrow", C.c_int32), ("order", C.c_int32),
+                ("candidates", C.c_int32), ("tune_ms", C.c_float), ("best_us", C.c_float)]
 
 
 class OgsInfo(C.Structure):
@@ -41,6 +50,10 @@ SIGNATURES = {
     "ogs_hip_scatter": (C.c_int, [_P, C.c_int, C.c_int, _I32, _P, _P, _P]),
     "ogs_hip_extract": (C.c_int, [C.c_int, _I32, _I32, _P, _P, _P, _P]),
     "ogs_hip_plan_info": (C.c_int, [_P, C.POINTER(PlanInfo)]),
+    "ogs_hip_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(BatchDesc), _I32, C.c_int, _P]),
+    "ogs_hip_batch_run": (C.c_int, [_P, _P]),
+    "ogs_hip_batch_destroy": (C.c_int, [_P]),
+    "ogs_hip_batch_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
     "ogs_setup": (C.c_int, [C.POINTER(_P), _I32, _P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ogs_free": (C.c_int, [_P]),
     "ogs_get_info": (C.c_int, [_P, C.POINTER(OgsInfo)]),

@@ -34,8 +34,11 @@
 #include <cfloat>
 #include <climits>
 #include <cstddef>
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <new>
+#include <vector>
 
 #include "ogs_hip.h"
 #include "ogs_internal.hpp"
@@ -94,10 +97,9 @@ struct Extent {
   int s0, nS;  // scatter list nonzeros [s0, s0+nS)
 };
 
-__device__ __forceinline__ bool block_extent(const int4 *__restrict__ part, int nrows,
+__device__ __forceinline__ bool block_extent(const int4 *__restrict__ part, int b, int nrows,
                                              const int32_t *__restrict__ gS, int gUseN,
                                              const int32_t *__restrict__ sS, int sUseN, Extent &e) {
-  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int4 p0 = part[b];
   const int4 p1 = part[b + 1];
   const int r1 = min(p1.x, nrows);
@@ -192,16 +194,13 @@ __device__ __forceinline__ void stage_rows(const int (&v)[kRowSlots], int nr, in
 // marks its heads from the row starts.  Empty rows have nothing to gather or
 // write, so they need no mark.
 template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_gs_sym_k1(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
-                                                      T *__restrict__ q) {
-  __shared__ T val[kCap];
-  __shared__ uint8_t head[kCap];
-
+__device__ __forceinline__ void gs_sym_k1_block(const int4 *__restrict__ part, int b, int nrows, const ListView &G,
+                                                int gUseN, T *__restrict__ q, T *__restrict__ val,
+                                                uint8_t *__restrict__ head) {
   Extent e;
-  if (!block_extent(part, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
+  if (!block_extent(part, b, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
   const int n = e.nG;
   if (n == 0) return;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
 
   uint32_t w[kSlots];
@@ -261,6 +260,14 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_k1(const int4 *__restrict__ p
   }
 }
 
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_k1(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
+                                                      T *__restrict__ q) {
+  __shared__ T val[kCap];
+  __shared__ uint8_t head[kCap];
+  gs_sym_k1_block<T, OP>(part, xcd_block(blockIdx.x, gridDim.x), nrows, G, gUseN, q, val, head);
+}
+
 // GatherScatter, K = 1.  SYM: gather and scatter lists are the same array
 // (Sym, or an Unsigned operator): the scatter reuses the gather's slots.
 template <typename T, int OP, bool SYM>
@@ -271,9 +278,9 @@ __global__ __launch_bounds__(kBlock) void k_gs_k1(const int4 *__restrict__ part,
   __shared__ int32_t rsG[kWMax + 1];
   __shared__ int32_t rsS[SYM ? 1 : kWMax + 1];
 
-  Extent e;
-  if (!block_extent(part, nrows, G.starts, gUseN, S.starts, sUseN, e)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  Extent e;
+  if (!block_extent(part, b, nrows, G.starts, gUseN, S.starts, sUseN, e)) return;
   const int tid = threadIdx.x;
 
   uint32_t w[kSlots], sw[SYM ? 1 : kSlots];
@@ -326,9 +333,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_k1(const int4 *__restrict__ p
                                                       const T *__restrict__ q, T *__restrict__ gq) {
   __shared__ T gval[kCap];
   __shared__ int32_t rsG[kWMax + 1];
-  Extent e;
-  if (!block_extent(part, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  Extent e;
+  if (!block_extent(part, b, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
   const int tid = threadIdx.x;
   uint32_t w[kSlots];
   int4 gb;
@@ -360,9 +367,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter_k1(const int4 *__restrict__ 
                                                        const T *__restrict__ gq, T *__restrict__ q) {
   __shared__ T sval[kCap];
   __shared__ int32_t rsS[kWMax + 1];
-  Extent e;
-  if (!block_extent(part, nrows, S.starts, sUseN, S.starts, sUseN, e)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  Extent e;
+  if (!block_extent(part, b, nrows, S.starts, sUseN, S.starts, sUseN, e)) return;
   const int tid = threadIdx.x;
   uint32_t w[kSlots];
   int4 sb;
@@ -397,17 +404,13 @@ __global__ __launch_bounds__(kBlock) void k_scatter_k1(const int4 *__restrict__ 
 // Flat index f = i*kw + k runs over (nonzero, field) pairs, so consecutive
 // lanes read consecutive fields of one node.
 template <typename T, int OP, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_gs_kv(const int4 *__restrict__ part, int nrows,
-                                                  const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
-                                                  const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
-                                                  int K, int kc, T *__restrict__ q) {
-  __shared__ T gval[kCap];
-  __shared__ T sval[SYM ? 1 : kCap];
-  __shared__ int32_t rsG[kWMax + 1];
-  __shared__ int32_t rsS[SYM ? 1 : kWMax + 1];
-
+__device__ __forceinline__ void gs_kv_block(const int4 *__restrict__ part, int b, int nrows,
+                                            const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
+                                            const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
+                                            int K, int kc, T *__restrict__ q, T *__restrict__ gval, T *__restrict__ sval,
+                                            int32_t *__restrict__ rsG, int32_t *__restrict__ rsS) {
   Extent e;
-  if (!block_extent(part, nrows, gS, gUseN, sS, sUseN, e)) return;
+  if (!block_extent(part, b, nrows, gS, gUseN, sS, sUseN, e)) return;
   const int tid = threadIdx.x;
 
 #pragma unroll
@@ -484,6 +487,52 @@ __global__ __launch_bounds__(kBlock) void k_gs_kv(const int4 *__restrict__ part,
   }
 }
 
+template <typename T, int OP, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_gs_kv(const int4 *__restrict__ part, int nrows,
+                                                  const int32_t *__restrict__ gS, const int32_t *__restrict__ gI, int gUseN,
+                                                  const int32_t *__restrict__ sS, const int32_t *__restrict__ sI, int sUseN,
+                                                  int K, int kc, T *__restrict__ q) {
+  __shared__ T gval[kCap];
+  __shared__ T sval[SYM ? 1 : kCap];
+  __shared__ int32_t rsG[kWMax + 1];
+  __shared__ int32_t rsS[SYM ? 1 : kWMax + 1];
+  gs_kv_block<T, OP, SYM>(part, xcd_block(blockIdx.x, gridDim.x), nrows, gS, gI, gUseN, sS, sI, sUseN, K, kc, q,
+                          gval, sval, rsG, rsS);
+}
+
+// ---------------------------------------------------------------------------
+// Batched GatherScatter Sym: one persistent launch over the blocks of many
+// plans.  A work item is (descriptor, block); workgroup w takes items w,
+// w + grid, ...  K = 1 items run the column-sorted path, K > 1 items the
+// field-pass path.  LDS is shared between the two layouts.
+struct BatchDev {
+  const int4 *part;
+  ListView G;
+  int nrows;
+  int K;
+  int kc;
+  void *q;
+};
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_batch_gs(const BatchDev *__restrict__ descs, const int2 *__restrict__ items,
+                                                     int nitems) {
+  constexpr int kAux = (kWMax + 1) * 4 > kCap ? (kWMax + 1) * 4 : kCap;  // max(rsG, head) bytes
+  __shared__ T vals[kCap];
+  __shared__ __align__(16) unsigned char aux[kAux];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int2 w = items[it];
+    const BatchDev d = descs[w.x];
+    if (d.K == 1) {
+      gs_sym_k1_block<T, OP>(d.part, w.y, d.nrows, d.G, 0, static_cast<T *>(d.q), vals, aux);
+    } else {
+      gs_kv_block<T, OP, true>(d.part, w.y, d.nrows, d.G.starts, d.G.ids, 0, d.G.starts, d.G.ids, 0, d.K, d.kc,
+                               static_cast<T *>(d.q), vals, nullptr, reinterpret_cast<int32_t *>(aux), nullptr);
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Gather: gv[k + row*K] = OP over the row's list.  Rows [0, nrows).
 template <typename T, int OP>
@@ -494,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int4 *__restrict__ part
   __shared__ int32_t rsG[kWMax + 1];
 
   Extent e;
-  if (!block_extent(part, nrows, gS, gUseN, gS, gUseN, e)) return;
+  if (!block_extent(part, xcd_block(blockIdx.x, gridDim.x), nrows, gS, gUseN, gS, gUseN, e)) return;
   const int tid = threadIdx.x;
 
 #pragma unroll
@@ -547,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int4 *__restrict__ par
   __shared__ int32_t rsS[kWMax + 1];
 
   Extent e;
-  if (!block_extent(part, nrows, sS, sUseN, sS, sUseN, e)) return;
+  if (!block_extent(part, xcd_block(blockIdx.x, gridDim.x), nrows, sS, sUseN, sS, sUseN, e)) return;
   const int tid = threadIdx.x;
 
 #pragma unroll
@@ -1146,6 +1195,7 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->widthK1 = p->nparts ? p->partT[0].W : 0;
   info->packedK1 = p->packTT.packed;
   info->plainK1 = p->packTT.plain;
+  info->Ncols = p->Ncols;
   return OGS_SUCCESS;
 }
 
@@ -1195,6 +1245,275 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids, const 
     case OGS_INT32: return launch_extract_typed<int32_t>(N, K, d_ids, d_q, d_gq, st);
     default: return launch_extract_typed<int64_t>(N, K, d_ids, d_q, d_gq, st);
   }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batch construction, tuning and graph capture.
+
+namespace ogs_hip {
+
+struct BatchConfig {
+  int grid;
+  int narrow;  // K > 1 descriptors use the next narrower block width
+  int order;   // 0: largest items first, 1: descriptor order
+};
+
+struct BatchTables {
+  std::vector<BatchDev> descs;
+  std::vector<int2> items;
+};
+
+static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
+                        const BatchConfig &c, BatchTables &t) {
+  t.descs.clear();
+  t.items.clear();
+  std::vector<int64_t> cost;
+  for (size_t i = 0; i < in.size(); ++i) {
+    const Plan *p = in[i].plan;
+    const int K = in[i].K;
+    BatchDev d = {};
+    d.q = qs[i];
+    d.K = K;
+    d.kc = K;
+    d.nrows = p->NrowsT;
+    const Partition *pt = nullptr;
+    if (p->NrowsT > 0 && p->nparts > 0) {
+      int kc = 1;
+      if (K == 1) {
+        pt = &p->partT[0];
+      } else {
+        pt = choose(p->partT, p->nparts, K, [](const Partition &x) { return x.maxT; }, &kc);
+        if (!pt) return OGS_E_ROW_TOO_LONG;
+        const int j = (int)(pt - p->partT);
+        if (c.narrow && kc == K && j + 1 < p->nparts) pt = &p->partT[j + 1];
+      }
+      d.kc = kc;
+      d.part = pt->d;
+      d.G = list_view(p, false, pt, false);
+    }
+    t.descs.push_back(d);
+    if (pt) {
+      for (int b = 0; b < pt->nblocks; ++b) {
+        t.items.push_back(make_int2((int)i, b));
+        cost.push_back((int64_t)pt->W * K);
+      }
+    }
+  }
+  if (c.order == 0) {
+    std::vector<size_t> idx(t.items.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<int2> sorted(idx.size());
+    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = t.items[idx[i]];
+    t.items.swap(sorted);
+  }
+  return OGS_SUCCESS;
+}
+
+template <typename T>
+static int launch_batch_typed(int op, const BatchDev *d, const int2 *items, int n, int grid, hipStream_t st) {
+  if (n == 0) return OGS_SUCCESS;
+  switch (op) {
+    case OGS_ADD: hipLaunchKernelGGL((k_batch_gs<T, OGS_ADD>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
+    case OGS_MUL: hipLaunchKernelGGL((k_batch_gs<T, OGS_MUL>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
+    case OGS_MAX: hipLaunchKernelGGL((k_batch_gs<T, OGS_MAX>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
+    case OGS_MIN: hipLaunchKernelGGL((k_batch_gs<T, OGS_MIN>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+static int launch_batch(int type, int op, const BatchDev *d, const int2 *items, int n, int grid, hipStream_t st) {
+  switch (type) {
+    case OGS_FLOAT: return launch_batch_typed<float>(op, d, items, n, grid, st);
+    case OGS_DOUBLE: return launch_batch_typed<double>(op, d, items, n, grid, st);
+    case OGS_INT32: return launch_batch_typed<int32_t>(op, d, items, n, grid, st);
+    case OGS_INT64: return launch_batch_typed<int64_t>(op, d, items, n, grid, st);
+    default: return OGS_E_ARG;
+  }
+}
+
+// Device copies of a batch's tables.
+struct BatchDevTables {
+  BatchDev *d = nullptr;
+  int2 *items = nullptr;
+  int n = 0;
+  void release() {
+    if (d) (void)hipFree(d);
+    if (items) (void)hipFree(items);
+    d = nullptr;
+    items = nullptr;
+    n = 0;
+  }
+};
+
+static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &out) {
+  out.release();
+  hipError_t e = hipMalloc(&out.d, sizeof(BatchDev) * std::max<size_t>(t.descs.size(), 1));
+  if (e == hipSuccess) e = hipMalloc(&out.items, sizeof(int2) * std::max<size_t>(t.items.size(), 1));
+  if (e == hipSuccess && !t.descs.empty())
+    e = hipMemcpyAsync(out.d, t.descs.data(), sizeof(BatchDev) * t.descs.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !t.items.empty())
+    e = hipMemcpyAsync(out.items, t.items.data(), sizeof(int2) * t.items.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  out.n = (int)t.items.size();
+  return (int)e;
+}
+
+}  // namespace ogs_hip
+
+struct ogs_hip_batch_s {
+  int type = 0, op = 0;
+  ogs_hip::BatchDevTables tables;
+  ogs_hip::BatchConfig cfg = {0, 0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int candidates = 0;
+  float tune_ms = 0.f, best_us = 0.f;
+};
+
+namespace ogs_hip {
+
+static size_t type_size(int type) {
+  return type == OGS_FLOAT || type == OGS_INT32 ? 4 : 8;
+}
+
+static void free_batch(ogs_hip_batch_s *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->tables.release();
+  delete b;
+}
+
+// Times a configuration on scratch buffers: 1 warm-up, 3 timed launches.
+static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
+                       const BatchConfig &c, hipStream_t st, float *us) {
+  BatchTables t;
+  int rc = batch_tables(in, qs, c, t);
+  if (rc) return rc;
+  BatchDevTables dt;
+  rc = upload_tables(t, st, dt);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) rc = (int)hipEventCreate(&e0);
+  if (!rc) rc = (int)hipEventCreate(&e1);
+  if (!rc) rc = launch_batch(type, op, dt.d, dt.items, dt.n, c.grid, st);
+  if (!rc) rc = (int)hipEventRecord(e0, st);
+  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt.d, dt.items, dt.n, c.grid, st);
+  if (!rc) rc = (int)hipEventRecord(e1, st);
+  if (!rc) rc = (int)hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (!rc) rc = (int)hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dt.release();
+  *us = ms * 1e3f / 3.f;
+  return rc;
+}
+
+}  // namespace ogs_hip
+
+extern "C" {
+
+int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip_batch_desc_t *descs, int32_t n,
+                         int tune, void *stream) {
+  using namespace ogs_hip;
+  if (!batch) return OGS_E_ARG;
+  *batch = nullptr;
+  if (bad_type(type) || bad_op(op) || n < 0 || (n > 0 && !descs)) return OGS_E_ARG;
+  std::vector<ogs_hip_batch_desc_t> in(descs, descs + n);
+  std::vector<void *> qs(n);
+  bool anyKv = false;
+  for (int i = 0; i < n; ++i) {
+    if (!in[i].plan || in[i].K < 1 || (!in[i].d_v && in[i].plan->Ncols > 0)) return OGS_E_ARG;
+    qs[i] = in[i].d_v;
+    anyKv |= in[i].K > 1;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  ogs_hip_batch_s *b = new (std::nothrow) ogs_hip_batch_s();
+  if (!b) return (int)hipErrorOutOfMemory;
+  b->type = type;
+  b->op = op;
+  int dev = 0, cus = 256;
+  int rc = (int)hipGetDevice(&dev);
+  if (!rc) rc = (int)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  BatchConfig best = {cus * 6, 0, 0};
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!rc && tune && n > 0) {
+    // scratch copies: tuning launches must not touch the caller's data
+    std::vector<void *> scratch(n, nullptr);
+    for (int i = 0; i < n && !rc; ++i) {
+      const size_t bytes = (size_t)in[i].plan->Ncols * in[i].K * type_size(type);
+      if (!bytes) continue;
+      rc = (int)hipMalloc(&scratch[i], bytes);
+      if (!rc) rc = (int)hipMemcpyAsync(scratch[i], in[i].d_v, bytes, hipMemcpyDeviceToDevice, st);
+    }
+    float bestUs = 0.f;
+    for (int g : {cus * 4, cus * 6, cus * 8}) {
+      for (int narrow = 0; narrow <= (anyKv ? 1 : 0); ++narrow) {
+        for (int order = 0; order <= 1; ++order) {
+          if (rc) break;
+          const BatchConfig c = {g, narrow, order};
+          float us = 0.f;
+          rc = time_config(type, op, in, scratch, c, st, &us);
+          ++b->candidates;
+          if (!rc && (bestUs == 0.f || us < bestUs)) {
+            bestUs = us;
+            best = c;
+          }
+        }
+      }
+    }
+    for (void *q : scratch)
+      if (q) (void)hipFree(q);
+    b->best_us = bestUs;
+  }
+  b->tune_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  b->cfg = best;
+  BatchTables t;
+  if (!rc) rc = batch_tables(in, qs, best, t);
+  if (!rc) rc = upload_tables(t, st, b->tables);
+  // capture the launch in a graph (capture needs a stream of its own)
+  hipStream_t cs = nullptr;
+  if (!rc) rc = (int)hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (!rc) rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (!rc) {
+    const int lr = launch_batch(type, op, b->tables.d, b->tables.items, b->tables.n, best.grid, cs);
+    const int er = (int)hipStreamEndCapture(cs, &b->graph);
+    rc = lr ? lr : er;
+  }
+  if (!rc) rc = (int)hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (cs) (void)hipStreamDestroy(cs);
+  if (rc) {
+    free_batch(b);
+    return rc;
+  }
+  *batch = b;
+  return OGS_SUCCESS;
+}
+
+int ogs_hip_batch_run(ogs_hip_batch_t batch, void *stream) {
+  if (!batch || !batch->exec) return OGS_E_ARG;
+  return (int)hipGraphLaunch(batch->exec, static_cast<hipStream_t>(stream));
+}
+
+int ogs_hip_batch_destroy(ogs_hip_batch_t batch) {
+  if (!batch) return OGS_E_ARG;
+  ogs_hip::free_batch(batch);
+  return OGS_SUCCESS;
+}
+
+int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info) {
+  if (!batch || !info) return OGS_E_ARG;
+  info->items = batch->tables.n;
+  info->grid = batch->cfg.grid;
+  info->narrow = batch->cfg.narrow;
+  info->order = batch->cfg.order;
+  info->candidates = batch->candidates;
+  info->tune_ms = batch->tune_ms;
+  info->best_us = batch->best_us;
+  return OGS_SUCCESS;
 }
 
 }  // extern "C"

@@ -97,6 +97,49 @@ class Plan:
             pass
 
 
+class Batch:
+    """ogs_hip_batch_t: GatherScatter Sym of many (plan, K, buffer) triples
+    in one persistent launch captured in a HIP graph.  Buffers are bound at
+    creation; tune=True times launch shapes on scratch copies first.  The
+    plans' owners (ogs_t objects) must outlive the batch."""
+
+    def __init__(self, items, op: int = Add, tune: bool = True, stream=None):
+        items = list(items)
+        if not items:
+            raise ValueError("empty batch")
+        ty = type_of(items[0][2])
+        descs = (_lib.BatchDesc * len(items))()
+        self._keep = []
+        for i, (plan, K, v) in enumerate(items):
+            if type_of(v) != ty:
+                raise TypeError("all buffers of a batch share one dtype")
+            if v.numel() < plan.info()["Ncols"] * K:
+                raise ValueError("buffer shorter than Ncols*K")
+            descs[i].plan = plan.h
+            descs[i].K = K
+            descs[i].d_v = _dev_ptr(v)
+            self._keep.append((plan, v))
+        h = C.c_void_p()
+        check(lib().ogs_hip_batch_create(C.byref(h), ty, op, descs, len(items), int(tune), _stream(stream)),
+              "ogs_hip_batch_create")
+        self.h = h
+
+    def run(self, stream=None) -> None:
+        check(lib().ogs_hip_batch_run(self.h, _stream(stream)), "ogs_hip_batch_run")
+
+    def info(self) -> dict:
+        i = _lib.BatchInfo()
+        check(lib().ogs_hip_batch_info(self.h, C.byref(i)), "ogs_hip_batch_info")
+        return {n: getattr(i, n) for n, _ in i._fields_}
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().ogs_hip_batch_destroy(self.h)
+        except Exception:
+            pass
+
+
 def extract(gq, q, ids, k, stream=None):
     """OKL extract (ogsKernels.okl:167-177): gq[k + n*K] = q[k + ids[n]*K]."""
     check(lib().ogs_hip_extract(type_of(q), ids.numel(), k, _dev_ptr(ids), _dev_ptr(q), _dev_ptr(gq),

@@ -111,9 +111,38 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * order (packedK1) or in CSR order (plainK1). */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
-  int32_t nblocksK1, widthK1, packedK1, plainK1;
+  int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
+
+/* Batched GatherScatter: many plans (meshes), each with its own K
+ * interleaved fields, in one persistent launch captured in a HIP graph.
+ * No counterpart in the reference (it launches one kernel per operator and
+ * field set).  Every descriptor runs GatherScatter Sym with the batch's
+ * type and op on its buffer d_v (Ncols*K values).  Buffers are bound at
+ * creation.  With tune != 0 the creation times candidate launch shapes
+ * (grid size, block width for K > 1, work order) on scratch copies of the
+ * buffers and keeps the fastest; the buffers themselves are not touched.
+ * Creation allocates and synchronises; ogs_hip_batch_run only launches the
+ * graph.  Concurrent runs of one batch race on its buffers. */
+typedef struct {
+  ogs_hip_plan_t plan;
+  int32_t K;
+  void *d_v;
+} ogs_hip_batch_desc_t;
+typedef struct ogs_hip_batch_s *ogs_hip_batch_t;
+int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip_batch_desc_t *descs,
+                         int32_t n, int tune, void *stream);
+int ogs_hip_batch_run(ogs_hip_batch_t batch, void *stream);
+int ogs_hip_batch_destroy(ogs_hip_batch_t batch);
+/* What the creation chose: work items, persistent grid, whether K>1
+ * descriptors use the next narrower block width, work order (0 = largest
+ * items first, 1 = descriptor order), and the tuning time / best run. */
+typedef struct {
+  int32_t items, grid, narrow, order, candidates;
+  float tune_ms, best_us;
+} ogs_hip_batch_info_t;
+int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info);
 
 /* ------------------------------------------------------------------ */
 /* 2. Handle layer: ogs_t at one rank                                  */
