@@ -28,6 +28,7 @@ constexpr uint32_t kPackPosMask = (1u << kPackPosBits) - 1;
 constexpr uint32_t kPackOffMask = (1u << kPackOffBits) - 1;
 static_assert(kPackSegShift == 30, "pack word is 2 + 18 + 1 + 11 bits");
 constexpr int kPackSegs = 4;
+constexpr int kPackParts = 5;  // T-list packs for partitions 0..4 (K up to 16 in one pass)
 static_assert(kCap <= (1 << kPackPosBits), "CSR positions of a block must fit the pack word");
 
 // One merge-path row partition: nblocks+1 entries {row, rowStartsT[row],
@@ -63,10 +64,11 @@ struct ogs_hip_plan_s {
   int nparts = 0;
   ogs_hip::Partition partT[ogs_hip::kMaxParts];  // cut on T, rows [0, NrowsT)
   ogs_hip::Partition partN[ogs_hip::kMaxParts];  // cut on N, rows [0, NrowsN)
-  // column-sorted packs for the K=1 kernels (partition 0 only)
-  ogs_hip::Pack packTT;  // T list over partT[0]
-  ogs_hip::Pack packNT;  // N list over partT[0]
-  ogs_hip::Pack packNN;  // N list over partN[0]
+  // column-sorted packs: the T list over partT[0..kPackParts) (K=1 and the
+  // K-field Sym kernels), the N list over partT[0] and over partN[0] (K=1)
+  ogs_hip::Pack packT[ogs_hip::kPackParts];
+  ogs_hip::Pack packNT;
+  ogs_hip::Pack packNN;
 };
 
 namespace ogs_hip {

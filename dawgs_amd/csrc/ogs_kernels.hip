@@ -268,6 +268,118 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_k1(const int4 *__restrict__ p
   gs_sym_k1_block<T, OP>(part, xcd_block(blockIdx.x, gridDim.x), nrows, G, gUseN, q, val, head);
 }
 
+// GatherScatter Sym with KT interleaved fields (KT a compile-time count).
+// Lanes walk (entry, field) pairs, entries in the pack's column order, so a
+// wave reads whole nodes' fields back to back; values land in LDS at
+// pos*KT + k and each head lane reduces one field of its row in CSR order.
+// The block holds at most kCap/KT entries (its partition is chosen so).
+// KT == 0: the field count is the runtime K, divided by with kmagic =
+// floor(2^32/K) + 1, which is exact for quotients of numbers below 2^16 by
+// K in [2, 2^16); K = 1 divides by itself.  The batched kernel uses this form
+// for every K, K = 1 included.
+template <int KT>
+struct FieldDiv {
+  int K;
+  uint32_t magic;
+  __device__ __forceinline__ int k() const { return KT ? KT : K; }
+  __device__ __forceinline__ int div(int f) const {
+    return KT ? f / KT : (K == 1 ? f : (int)__umulhi((uint32_t)f, magic));
+  }
+};
+
+template <typename T, int OP, int KT>
+__device__ __forceinline__ void gs_sym_kv_block(const int4 *__restrict__ part, int b, int nrows, const ListView &G,
+                                                int gUseN, T *__restrict__ q, T *__restrict__ val,
+                                                uint8_t *__restrict__ head, FieldDiv<KT> fd = {KT, 0}) {
+  Extent e;
+  if (!block_extent(part, b, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
+  const int n = e.nG;
+  if (n == 0) return;
+  const int KF = fd.k();
+  const int nv = n * KF;
+  const int tid = threadIdx.x;
+  int4 bb = make_int4(-1, 0, 0, 0);
+  if (G.pk && e.full) bb = G.base[b];
+  const bool packed = bb.x >= 0;
+  const uint32_t *src = packed ? G.pk + e.g0 : reinterpret_cast<const uint32_t *>(G.ids) + e.g0;
+  uint32_t w[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int f = tid + j * kBlock;
+    w[j] = f < nv ? __builtin_nontemporal_load(src + fd.div(f)) : 0u;
+  }
+  int rg[kRowSlots];
+  if (!packed) row_words(G.starts, e.r0, e.nr, rg);
+  T x[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int f = tid + j * kBlock;
+    x[j] = q[(size_t)slot_col(w[j], bb, packed) * KF + (f - fd.div(f) * KF)];
+  }
+  if (packed) {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nv) {
+        const int k = f - fd.div(f) * KF;
+        const int ps = (int)(w[j] & kPackPosMask);
+        val[ps * KF + k] = x[j];
+        if (k == 0) head[ps] = (uint8_t)((w[j] >> kPackHeadBit) & 1u);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int f = tid + j * kBlock;
+      if (f < nv) {
+        val[f] = x[j];
+        if (f - fd.div(f) * KF == 0) head[fd.div(f)] = 0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRowSlots; ++j) {
+      const int r = tid + j * kBlock;
+      const int a = rg[j] - e.g0;
+      if (r < e.nr && a < n) head[a] = 1;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < nv; t += kBlock) {
+    const int i = fd.div(t), k = t - i * KF;
+    if (head[i]) {
+      T acc = Red<T, OP>::init();
+      int m = i;
+      do {
+        Red<T, OP>::apply(acc, val[m * KF + k]);
+        ++m;
+      } while (m < n && !head[m]);
+      for (int r = i; r < m; ++r) val[r * KF + k] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int f = tid + j * kBlock;
+    if (f < nv) {
+      const int k = f - fd.div(f) * KF;
+      const int ps = packed ? (int)(w[j] & kPackPosMask) : fd.div(f);
+      q[(size_t)slot_col(w[j], bb, packed) * KF + k] = val[ps * KF + k];
+    }
+  }
+}
+
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_kv(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
+                                                      T *__restrict__ q) {
+  __shared__ T val[kCap];
+  __shared__ uint8_t head[kCap];
+  gs_sym_kv_block<T, OP, KT>(part, xcd_block(blockIdx.x, gridDim.x), nrows, G, gUseN, q, val, head);
+}
+
+// Field counts with a compile-time kernel; other K use the field-pass kernel.
+__host__ __device__ constexpr bool packed_k(int K) { return K == 2 || K == 3 || K == 4 || K == 8 || K == 16; }
+
 // GatherScatter, K = 1.  SYM: gather and scatter lists are the same array
 // (Sym, or an Unsigned operator): the scatter reuses the gather's slots.
 template <typename T, int OP, bool SYM>
@@ -511,24 +623,59 @@ struct BatchDev {
   int nrows;
   int K;
   int kc;
+  uint32_t kmagic;  // floor(2^32/K) + 1 for K >= 2
   void *q;
 };
 
+// Pointers read from descriptor memory are generic to the compiler, which
+// would then access them with flat instructions (counted on the LDS counter
+// too, so every wait also drains LDS traffic).  They all point to device
+// memory: casting through the global address space lets the compiler use
+// global loads and stores.
+template <typename P>
+__device__ __forceinline__ P *as_global(P *p) {
+  return (P *)(__attribute__((address_space(1))) P *)(uintptr_t)p;
+}
+
+__device__ __forceinline__ BatchDev load_desc(const BatchDev *descs, int i) {
+  BatchDev d = descs[i];
+  d.part = as_global(d.part);
+  d.G.starts = as_global(d.G.starts);
+  d.G.ids = as_global(d.G.ids);
+  d.G.pk = as_global(d.G.pk);
+  d.G.base = as_global(d.G.base);
+  d.q = as_global(d.q);
+  return d;
+}
+
+// The batched kernel has one path, K fields in one pass for any K (K = 1
+// included; the division by K uses the descriptor's magic multiplier), so
+// its register budget is that of one path.  Descriptors whose fields take
+// several passes run in k_batch_gs_rest, a second node of the same graph.
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_batch_gs(const BatchDev *__restrict__ descs, const int2 *__restrict__ items,
                                                      int nitems) {
-  constexpr int kAux = (kWMax + 1) * 4 > kCap ? (kWMax + 1) * 4 : kCap;  // max(rsG, head) bytes
   __shared__ T vals[kCap];
-  __shared__ __align__(16) unsigned char aux[kAux];
+  __shared__ uint8_t head[kCap];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int2 w = items[it];
-    const BatchDev d = descs[w.x];
-    if (d.K == 1) {
-      gs_sym_k1_block<T, OP>(d.part, w.y, d.nrows, d.G, 0, static_cast<T *>(d.q), vals, aux);
-    } else {
-      gs_kv_block<T, OP, true>(d.part, w.y, d.nrows, d.G.starts, d.G.ids, 0, d.G.starts, d.G.ids, 0, d.K, d.kc,
-                               static_cast<T *>(d.q), vals, nullptr, reinterpret_cast<int32_t *>(aux), nullptr);
-    }
+    const BatchDev d = load_desc(descs, w.x);
+    gs_sym_kv_block<T, OP, 0>(d.part, w.y, d.nrows, d.G, 0, static_cast<T *>(d.q), vals, head,
+                              FieldDiv<0>{d.K, d.kmagic});
+    __syncthreads();
+  }
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_batch_gs_rest(const BatchDev *__restrict__ descs,
+                                                          const int2 *__restrict__ items, int nitems) {
+  __shared__ T gval[kCap];
+  __shared__ int32_t rsG[kWMax + 1];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int2 w = items[it];
+    const BatchDev d = load_desc(descs, w.x);
+    gs_kv_block<T, OP, true>(d.part, w.y, d.nrows, d.G.starts, d.G.ids, 0, d.G.starts, d.G.ids, 0, d.K, d.kc,
+                             static_cast<T *>(d.q), gval, nullptr, rsG, nullptr);
     __syncthreads();
   }
 }
@@ -874,9 +1021,14 @@ static ListView list_view(const Plan *p, bool listN, const Partition *pt, bool p
   L.pk = nullptr;
   L.base = nullptr;
   const Pack *pk = nullptr;
-  if (pt == (partIsN ? &p->partN[0] : &p->partT[0])) {
-    if (partIsN) pk = listN ? &p->packNN : nullptr;
-    else pk = listN ? &p->packNT : &p->packTT;
+  if (partIsN) {
+    if (pt == &p->partN[0] && listN) pk = &p->packNN;
+  } else {
+    const int j = (int)(pt - p->partT);
+    if (j >= 0 && j < kPackParts && j < p->nparts) {
+      if (!listN) pk = &p->packT[j];
+      else if (j == 0) pk = &p->packNT;
+    }
   }
   if (pk && pk->pk) {
     L.pk = pk->pk;
@@ -913,6 +1065,14 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   if (K == 1) {                                                                                                 \
     if (sym) hipLaunchKernelGGL((k_gs_sym_k1<T, OPV>), grid, block, 0, st, pt->d, nrows, G, gN, v);              \
     else hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);       \
+  } else if (sym && kc == K && packed_k(K)) {                                                                    \
+    switch (K) {                                                                                                \
+      case 2: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 2>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
+      case 3: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 3>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
+      case 4: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 4>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
+      case 8: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 8>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
+      default: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 16>), grid, block, 0, st, pt->d, nrows, G, gN, v); break; \
+    }                                                                                                           \
   } else {                                                                                                      \
     if (sym) hipLaunchKernelGGL((k_gs_kv<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v); \
     else hipLaunchKernelGGL((k_gs_kv<T, OPV, false>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v);   \
@@ -1073,11 +1233,11 @@ static void free_pack(Pack &k) {
 }
 
 static void free_plan(Plan *p) {
-  const bool nt = p->packNT.pk && p->packNT.pk != p->packTT.pk;
-  const bool nn = p->packNN.pk && p->packNN.pk != p->packTT.pk;
+  const bool nt = p->packNT.pk && p->packNT.pk != p->packT[0].pk;
+  const bool nn = p->packNN.pk && p->packNN.pk != p->packT[0].pk;
   if (nt) free_pack(p->packNT);
   if (nn) free_pack(p->packNN);
-  free_pack(p->packTT);
+  for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
   for (int j = 0; j < p->nparts; ++j) {
     if (p->partT[j].d) (void)hipFree(p->partT[j].d);
     if (p->partN[j].d && p->partN[j].d != p->partT[j].d) (void)hipFree(p->partN[j].d);
@@ -1162,10 +1322,11 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
   }
   // column-sorted packs of the K=1 partitions
   if (!rc && p->nparts > 0) {
-    rc = build_pack(p->partT[0], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packTT);
+    for (int j = 0; j < kPackParts && j < p->nparts && !rc; ++j)
+      rc = build_pack(p->partT[j], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packT[j]);
     if (!rc && p->aliased) {
-      p->packNT = p->packTT;
-      p->packNN = p->packTT;
+      p->packNT = p->packT[0];
+      p->packNN = p->packT[0];
     } else if (!rc) {
       rc = build_pack(p->partT[0], NrowsT, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNT);
       if (!rc) rc = build_pack(p->partN[0], NrowsN, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNN);
@@ -1193,8 +1354,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->maxRowN = p->maxRowN;
   info->nblocksK1 = p->nparts ? p->partT[0].nblocks : 0;
   info->widthK1 = p->nparts ? p->partT[0].W : 0;
-  info->packedK1 = p->packTT.packed;
-  info->plainK1 = p->packTT.plain;
+  info->packedK1 = p->packT[0].packed;
+  info->plainK1 = p->packT[0].plain;
   info->Ncols = p->Ncols;
   return OGS_SUCCESS;
 }
@@ -1262,13 +1423,15 @@ struct BatchConfig {
 
 struct BatchTables {
   std::vector<BatchDev> descs;
-  std::vector<int2> items;
+  std::vector<int2> items;  // K = 1 and one-pass K-field blocks (k_batch_gs)
+  std::vector<int2> rest;   // blocks of descriptors needing several field passes (k_batch_gs_rest)
 };
 
 static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
                         const BatchConfig &c, BatchTables &t) {
   t.descs.clear();
   t.items.clear();
+  t.rest.clear();
   std::vector<int64_t> cost;
   for (size_t i = 0; i < in.size(); ++i) {
     const Plan *p = in[i].plan;
@@ -1277,6 +1440,7 @@ static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::
     d.q = qs[i];
     d.K = K;
     d.kc = K;
+    d.kmagic = K >= 2 ? (uint32_t)((uint64_t(1) << 32) / (uint64_t)K + 1u) : 0u;
     d.nrows = p->NrowsT;
     const Partition *pt = nullptr;
     if (p->NrowsT > 0 && p->nparts > 0) {
@@ -1296,8 +1460,12 @@ static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::
     t.descs.push_back(d);
     if (pt) {
       for (int b = 0; b < pt->nblocks; ++b) {
-        t.items.push_back(make_int2((int)i, b));
-        cost.push_back((int64_t)pt->W * K);
+        if (d.kc == K) {
+          t.items.push_back(make_int2((int)i, b));
+          cost.push_back((int64_t)pt->W * K);
+        } else {
+          t.rest.push_back(make_int2((int)i, b));
+        }
       }
     }
   }
@@ -1313,41 +1481,47 @@ static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::
 }
 
 template <typename T>
-static int launch_batch_typed(int op, const BatchDev *d, const int2 *items, int n, int grid, hipStream_t st) {
-  if (n == 0) return OGS_SUCCESS;
+static int launch_batch_typed(int op, const BatchDev *d, const int2 *items, int n, const int2 *rest, int nrest,
+                              int grid, hipStream_t st) {
+#define OGS_B_LAUNCH(OPV)                                                                                         \
+  if (n) hipLaunchKernelGGL((k_batch_gs<T, OPV>), dim3(grid), dim3(kBlock), 0, st, d, items, n);                  \
+  if (nrest) hipLaunchKernelGGL((k_batch_gs_rest<T, OPV>), dim3(grid), dim3(kBlock), 0, st, d, rest, nrest);
   switch (op) {
-    case OGS_ADD: hipLaunchKernelGGL((k_batch_gs<T, OGS_ADD>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
-    case OGS_MUL: hipLaunchKernelGGL((k_batch_gs<T, OGS_MUL>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
-    case OGS_MAX: hipLaunchKernelGGL((k_batch_gs<T, OGS_MAX>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
-    case OGS_MIN: hipLaunchKernelGGL((k_batch_gs<T, OGS_MIN>), dim3(grid), dim3(kBlock), 0, st, d, items, n); break;
+    case OGS_ADD: OGS_B_LAUNCH(OGS_ADD) break;
+    case OGS_MUL: OGS_B_LAUNCH(OGS_MUL) break;
+    case OGS_MAX: OGS_B_LAUNCH(OGS_MAX) break;
+    case OGS_MIN: OGS_B_LAUNCH(OGS_MIN) break;
     default: return OGS_E_ARG;
   }
+#undef OGS_B_LAUNCH
   return (int)hipGetLastError();
-}
-
-static int launch_batch(int type, int op, const BatchDev *d, const int2 *items, int n, int grid, hipStream_t st) {
-  switch (type) {
-    case OGS_FLOAT: return launch_batch_typed<float>(op, d, items, n, grid, st);
-    case OGS_DOUBLE: return launch_batch_typed<double>(op, d, items, n, grid, st);
-    case OGS_INT32: return launch_batch_typed<int32_t>(op, d, items, n, grid, st);
-    case OGS_INT64: return launch_batch_typed<int64_t>(op, d, items, n, grid, st);
-    default: return OGS_E_ARG;
-  }
 }
 
 // Device copies of a batch's tables.
 struct BatchDevTables {
   BatchDev *d = nullptr;
   int2 *items = nullptr;
-  int n = 0;
+  int2 *rest = nullptr;
+  int n = 0, nrest = 0;
   void release() {
     if (d) (void)hipFree(d);
     if (items) (void)hipFree(items);
+    if (rest) (void)hipFree(rest);
     d = nullptr;
-    items = nullptr;
-    n = 0;
+    items = rest = nullptr;
+    n = nrest = 0;
   }
 };
+
+static int launch_batch(int type, int op, const BatchDevTables &t, int grid, hipStream_t st) {
+  switch (type) {
+    case OGS_FLOAT: return launch_batch_typed<float>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
+    case OGS_DOUBLE: return launch_batch_typed<double>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
+    case OGS_INT32: return launch_batch_typed<int32_t>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
+    case OGS_INT64: return launch_batch_typed<int64_t>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
+    default: return OGS_E_ARG;
+  }
+}
 
 static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &out) {
   out.release();
@@ -1355,10 +1529,14 @@ static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &o
   if (e == hipSuccess) e = hipMalloc(&out.items, sizeof(int2) * std::max<size_t>(t.items.size(), 1));
   if (e == hipSuccess && !t.descs.empty())
     e = hipMemcpyAsync(out.d, t.descs.data(), sizeof(BatchDev) * t.descs.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMalloc(&out.rest, sizeof(int2) * std::max<size_t>(t.rest.size(), 1));
   if (e == hipSuccess && !t.items.empty())
     e = hipMemcpyAsync(out.items, t.items.data(), sizeof(int2) * t.items.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !t.rest.empty())
+    e = hipMemcpyAsync(out.rest, t.rest.data(), sizeof(int2) * t.rest.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   out.n = (int)t.items.size();
+  out.nrest = (int)t.rest.size();
   return (int)e;
 }
 
@@ -1398,9 +1576,9 @@ static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t>
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
-  if (!rc) rc = launch_batch(type, op, dt.d, dt.items, dt.n, c.grid, st);
+  if (!rc) rc = launch_batch(type, op, dt, c.grid, st);
   if (!rc) rc = (int)hipEventRecord(e0, st);
-  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt.d, dt.items, dt.n, c.grid, st);
+  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt, c.grid, st);
   if (!rc) rc = (int)hipEventRecord(e1, st);
   if (!rc) rc = (int)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -1479,7 +1657,7 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!rc) rc = (int)hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (!rc) rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (!rc) {
-    const int lr = launch_batch(type, op, b->tables.d, b->tables.items, b->tables.n, best.grid, cs);
+    const int lr = launch_batch(type, op, b->tables, best.grid, cs);
     const int er = (int)hipStreamEndCapture(cs, &b->graph);
     rc = lr ? lr : er;
   }
@@ -1506,7 +1684,7 @@ int ogs_hip_batch_destroy(ogs_hip_batch_t batch) {
 
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info) {
   if (!batch || !info) return OGS_E_ARG;
-  info->items = batch->tables.n;
+  info->items = batch->tables.n + batch->tables.nrest;
   info->grid = batch->cfg.grid;
   info->narrow = batch->cfg.narrow;
   info->order = batch->cfg.order;
