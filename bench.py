@@ -14,6 +14,7 @@ definition: NgatherGlobal / time per call (dawgsMain.cpp:207).
 Prints ONE JSON line on stdout; diagnostics go to stderr.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 100] [--nx 48] [--p 7] [--no-cpu] [--copy-probe]
+    python bench.py --sweep [DEGREES] | --batched | --precision
 """
 from __future__ import annotations
 
@@ -213,6 +214,129 @@ def run_batched(args) -> int:
     return 0
 
 
+def error_stats(res: np.ndarray, y: np.ndarray) -> dict:
+    """Error of `res` against the fp64 result `y`: max absolute, normwise
+    relative ||res - y||_2 / ||y||_2, and max elementwise relative over
+    |y| > 1e-300."""
+    d = res.astype(np.float64) - y
+    big = np.abs(y) > 1e-300
+    return {"max_abs": float(np.max(np.abs(d))),
+            "normwise_rel": float(np.linalg.norm(d) / np.linalg.norm(y)),
+            "max_elem_rel": float(np.max(np.abs(d[big]) / np.abs(y[big])))}
+
+
+def run_precision(args) -> int:
+    """Precision form: FLOAT_C (float storage) and BF16 storage, Add reduced
+    in float with two-sum compensation, K=3 interleaved fields on the 48^3
+    p=7 mesh, values log-uniform in magnitude over [1e-3, 1e3] with random
+    sign.  Timed against the plain fp64 method (one K=1 launch per field, the
+    reference's layout) and fp64 K=3 in one launch; errors against the fp64
+    oracle of the unrounded values."""
+    import torch
+    import dawgs_amd as D
+    from dawgs_amd.mesh import make_ids
+    from dawgs_amd.perf import time_calls
+    from oracle import oracle as O  # the checker: fp64 results and bit-exact comparisons
+    nx, p, K = args.nx, args.p, 3
+    t0 = time.perf_counter()
+    ids = make_ids(nx, nx, nx, p)
+    N = len(ids)
+    ogs = D.ogs_t()
+    ogs.Setup(N, ids, D.Unsigned, D.Auto, False, False)
+    del ids
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    mag = torch.empty(N * K, dtype=torch.float64, device="cuda").uniform_(-3.0, 3.0, generator=gen)
+    sign = torch.randint(0, 2, (N * K,), device="cuda", generator=gen).to(torch.float64) * 2 - 1
+    x = torch.pow(10.0, mag) * sign
+    del mag, sign
+    x32 = x.to(torch.float32)
+    xb = x32.to(torch.bfloat16)  # storage rounding: fp64 -> float -> bf16, each to nearest even
+    log(f"setup {time.perf_counter() - t0:.1f} s: N={N}, K={K}, nnz={ogs.nnzT}")
+    fdofs = ogs.NgatherGlobal * K
+
+    # errors first, on fresh copies: one GatherScatter each
+    c = ogs.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=N)
+    xh = x.cpu().numpy()
+    y = ref.gather_scatter(xh.copy(), K, O.ADD, O.SYM)
+    errors = {}
+    for name, dev, ty in (("fp32_compensated", x32, D.FloatC), ("bf16", xb, D.BF16)):
+        d = dev.clone()
+        ogs.GatherScatter(d, K, D.Add, D.Sym, type=ty)
+        if ty == D.BF16:
+            got = d.view(torch.int16).cpu().numpy().view(np.uint16)
+            want = ref.gather_scatter(xb.view(torch.int16).cpu().numpy().view(np.uint16), K, O.ADD, O.SYM,
+                                      type=O.BF16)
+            res = O.bf16_to_float(got)
+        else:
+            got = d.cpu().numpy()
+            want = ref.gather_scatter(x32.cpu().numpy(), K, O.ADD, O.SYM, type=O.FLOAT_C)
+            res = got
+        st = error_stats(res, y)
+        st["bit_exact_vs_oracle"] = got.tobytes() == want.tobytes()
+        errors[name] = st
+        log(f"{name}: {st}")
+        del d, got, want, res
+    # the plain fp32 sum (reference arithmetic, no compensation) beside it
+    d = x32.clone()
+    ogs.GatherScatter(d, K, D.Add, D.Sym)
+    errors["fp32_plain"] = error_stats(d.cpu().numpy(), y)
+    log(f"fp32_plain: {errors['fp32_plain']}")
+    del d, y, xh, ref, c
+
+    fields = [x.view(N, K)[:, k].contiguous() for k in range(K)]  # plain method: one array per field
+    x64 = x.clone()
+    f32 = x32.clone()
+    bf = xb.clone()
+    b32 = D.Batch([(ogs.plan, K, f32)], D.Add, tune=True, type=D.FloatC)
+    bb = D.Batch([(ogs.plan, K, bf)], D.Add, tune=True, type=D.BF16)
+    b64 = D.Batch([(ogs.plan, K, x64)], D.Add, tune=True)
+
+    def plain64():
+        for f in fields:
+            ogs.GatherScatter(f, 1, D.Add, D.Sym)
+
+    forms = {
+        "fp64_plain_per_field": (plain64, 8, K),
+        "fp64_k3": (lambda: ogs.GatherScatter(x64, K, D.Add, D.Sym), 8, 1),
+        "fp64_k3_batched": (b64.run, 8, 1),
+        "fp32_compensated_k3": (lambda: ogs.GatherScatter(f32, K, D.Add, D.Sym, type=D.FloatC), 4, 1),
+        "fp32_compensated_k3_batched": (b32.run, 4, 1),
+        "bf16_k3": (lambda: ogs.GatherScatter(bf, K, D.Add, D.Sym), 2, 1),
+        "bf16_k3_batched": (bb.run, 2, 1),
+    }
+    res = {}
+    for name, (fn, s, launches) in forms.items():
+        t = time_calls(fn, args.warmup, args.steps)
+        kk = 1 if name == "fp64_plain_per_field" else K
+        B = launches * (2 * kk * ogs.nnzT * s + 4 * ogs.nnzT + 4 * (ogs.NrowsT + 1))
+        res[name] = {"us": round(t * 1e6, 1), "DOFSps": fdofs / t, "launches": launches,
+                     "algorithmic_bytes": B, "GBps": round(B / t / 1e9, 1),
+                     "frac": round(B / t / 1e9 / HBM_PEAK_GBS, 4)}
+    base = res["fp64_plain_per_field"]["DOFSps"]
+    for name, r in res.items():
+        r["speedup_vs_fp64_plain"] = round(r["DOFSps"] / base, 3)
+        r["speedup_vs_fp64_k3"] = round(r["DOFSps"] / res["fp64_k3"]["DOFSps"], 3)
+        log(f"{name:30s} {r['us']:8.1f} us  {r['DOFSps']:.4g} field-DOFS/s  {r['GBps']:7.1f} GB/s "
+            f"({100 * r['frac']:.1f}%)  x{r['speedup_vs_fp64_plain']:.2f} vs plain fp64")
+    head = res["fp32_compensated_k3_batched"]
+    out = {"metric": "precision form: batched FP32-compensated GatherScatter field-DOFS/s vs plain fp64 "
+                     "(48^3 p=7, K=3)",
+           "value": head["DOFSps"], "unit": "DOFS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": head["us"] * 1e-3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"ogs GatherScatter Add Sym, K={K} interleaved fields on a {nx}^3 hex mesh, "
+                                  f"p={p}; values 10^U[-3,3] with random sign (fp64, seed 0), stored as float "
+                                  f"/ bf16", "nx": nx, "p": p, "K": K, "local_dofs": N,
+                      "global_field_dofs": int(fdofs)},
+           "speedup_vs_fp64_plain": head["speedup_vs_fp64_plain"],
+           "speedup_vs_fp64_k3": head["speedup_vs_fp64_k3"],
+           "forms": res, "errors_vs_fp64": errors,
+           "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": head["frac"], "algorithmic_bytes": head["algorithmic_bytes"]}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +354,10 @@ def main() -> int:
                     help="batched form: 48 meshes (nx,ny,nz ~ U{4..32}, p ~ U{1..7}, K ~ {1,3,8,16}), fp64 "
                          "Add GatherScatter in one persistent graph-captured launch vs one plain launch per "
                          "mesh and field; one JSON line on stdout")
+    ap.add_argument("--precision", action="store_true",
+                    help="precision form: FP32-compensated and BF16 storage, K=3 fields on the 48^3 p=7 mesh, "
+                         "values log-uniform over [1e-3,1e3] with random sign; errors against the fp64 oracle "
+                         "and DOFS/s against the plain fp64 kernel; one JSON line on stdout")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, time a plain device copy of the same byte count "
                          "(the practical streaming ceiling, and a known-size kernel for scaling "
@@ -254,6 +382,8 @@ def main() -> int:
         return run_sweep(args)
     if args.batched:
         return run_batched(args)
+    if args.precision:
+        return run_precision(args)
     nx, p = args.nx, args.p
 
     t0 = time.perf_counter()

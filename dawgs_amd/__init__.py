@@ -4,11 +4,11 @@ The compute path is libogs_hip.so (gfx950 HIP kernels behind the C-ABI in
 include/ogs_hip.h); this package is its Python host mirror.  Build the
 library with ``python -m dawgs_amd.build``.
 """
-from .ogs import (Add, AllToAll, Auto, BF16, Batch, CrystalRouter, Double, Float, Halo, Int32, Int64, Max, Min,
+from .ogs import (Add, AllToAll, Auto, BF16, Batch, CrystalRouter, Double, Float, FloatC, Halo, Int32, Int64, Max, Min,
                   Mul, NoTrans, Pairwise, Plan, Signed, Sym, Trans, Unsigned, extract, ogs_t)
 from .mesh import make_ids
 
 __all__ = ["ogs_t", "Plan", "Batch", "extract", "make_ids",
-           "Float", "Double", "Int32", "Int64", "BF16", "Add", "Mul", "Max", "Min",
+           "Float", "Double", "Int32", "Int64", "BF16", "FloatC", "Add", "Mul", "Max", "Min",
            "Sym", "NoTrans", "Trans", "Auto", "Pairwise", "CrystalRouter", "AllToAll",
            "Unsigned", "Signed", "Halo"]

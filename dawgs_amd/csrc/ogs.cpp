@@ -137,6 +137,8 @@ OGS_INSTANTIATE(float)
 OGS_INSTANTIATE(double)
 OGS_INSTANTIATE(int32_t)
 OGS_INSTANTIATE(int64_t)
+OGS_INSTANTIATE(bfloat16)
+OGS_INSTANTIATE(float_c)
 #undef OGS_INSTANTIATE
 
 }  // namespace ogs
@@ -174,6 +176,8 @@ int by_type(int type, F &&f) {
     case OGS_DOUBLE: return f(double{});
     case OGS_INT32: return f(int32_t{});
     case OGS_INT64: return f(int64_t{});
+    case OGS_BF16: return f(ogs::bfloat16{});
+    case OGS_FLOAT_C: return f(ogs::float_c{});
     default: return OGS_E_ARG;
   }
 }

@@ -129,7 +129,9 @@ class platform_t {
 
 namespace ogs {
 
-typedef enum { Float, Double, Int32, Int64 } Type;             // ogs.hpp:179
+// ogs.hpp:179, plus the precision-form storage types (Bf16, FloatC: no
+// counterpart in the reference; see ogs_type_t in ogs_hip.h)
+typedef enum { Float, Double, Int32, Int64, Bf16, FloatC } Type;
 typedef enum { Add, Mul, Max, Min } Op;                        // ogs.hpp:191
 typedef enum { Sym, NoTrans, Trans } Transpose;                // ogs.hpp:194
 typedef enum { Auto, Pairwise, CrystalRouter, AllToAll } Method;  // ogs.hpp:197
@@ -140,6 +142,17 @@ template <> struct ogsType<float> { static constexpr Type get() { return Float; 
 template <> struct ogsType<double> { static constexpr Type get() { return Double; } };
 template <> struct ogsType<int32_t> { static constexpr Type get() { return Int32; } };
 template <> struct ogsType<int64_t> { static constexpr Type get() { return Int64; } };
+
+// Precision-form storage: a bfloat16 bit pattern, and a float whose Add
+// reduces with two-sum compensation.
+struct bfloat16 {
+  uint16_t bits;
+};
+struct float_c {
+  float v;
+};
+template <> struct ogsType<bfloat16> { static constexpr Type get() { return Bf16; } };
+template <> struct ogsType<float_c> { static constexpr Type get() { return FloatC; } };
 
 // The CSR "Z" operator (ogsOperator.hpp:36-132): host lists, their device
 // copies and the device plan that runs them.

@@ -57,6 +57,7 @@ struct Pack {
 // The opaque plan behind ogs_hip_plan_t.
 struct ogs_hip_plan_s {
   int device = 0;
+  int cus = 0;  // compute units of the device (persistent launch sizes)
   int32_t Ncols = 0, NrowsN = 0, NrowsT = 0, nnzN = 0, nnzT = 0;
   int32_t maxRowN = 0, maxRowT = 0;
   const int32_t *rsN = nullptr, *ciN = nullptr, *rsT = nullptr, *ciT = nullptr;

@@ -38,6 +38,7 @@
 #include <chrono>
 #include <cstdint>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "ogs_hip.h"
@@ -62,23 +63,121 @@ template <> __device__ __forceinline__ int32_t wrap_mul(int32_t a, int32_t b) { 
 template <> __device__ __forceinline__ int64_t wrap_add(int64_t a, int64_t b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
 template <> __device__ __forceinline__ int64_t wrap_mul(int64_t a, int64_t b) { return (int64_t)((uint64_t)a * (uint64_t)b); }
 
+// Red<T, OP>: A is the accumulator, init() its start value, apply() folds
+// one member in, fin() gives the value written back.  For the reference's
+// four types A is T and fin() is the identity.
 template <typename T, int OP> struct Red;
 template <typename T> struct Red<T, OGS_ADD> {
-  static __device__ __forceinline__ T init() { return T(0); }
-  static __device__ __forceinline__ void apply(T &a, T b) { a = wrap_add(a, b); }
+  using A = T;
+  static __device__ __forceinline__ A init() { return T(0); }
+  static __device__ __forceinline__ void apply(A &a, T b) { a = wrap_add(a, b); }
+  static __device__ __forceinline__ T fin(A a) { return a; }
 };
 template <typename T> struct Red<T, OGS_MUL> {
-  static __device__ __forceinline__ T init() { return T(1); }
-  static __device__ __forceinline__ void apply(T &a, T b) { a = wrap_mul(a, b); }
+  using A = T;
+  static __device__ __forceinline__ A init() { return T(1); }
+  static __device__ __forceinline__ void apply(A &a, T b) { a = wrap_mul(a, b); }
+  static __device__ __forceinline__ T fin(A a) { return a; }
 };
 template <typename T> struct Red<T, OGS_MAX> {
-  static __device__ __forceinline__ T init() { return -Lim<T>::max(); }
-  static __device__ __forceinline__ void apply(T &a, T b) { a = (b > a) ? b : a; }
+  using A = T;
+  static __device__ __forceinline__ A init() { return -Lim<T>::max(); }
+  static __device__ __forceinline__ void apply(A &a, T b) { a = (b > a) ? b : a; }
+  static __device__ __forceinline__ T fin(A a) { return a; }
 };
 template <typename T> struct Red<T, OGS_MIN> {
-  static __device__ __forceinline__ T init() { return Lim<T>::max(); }
-  static __device__ __forceinline__ void apply(T &a, T b) { a = (b < a) ? b : a; }
+  using A = T;
+  static __device__ __forceinline__ A init() { return Lim<T>::max(); }
+  static __device__ __forceinline__ void apply(A &a, T b) { a = (b < a) ? b : a; }
+  static __device__ __forceinline__ T fin(A a) { return a; }
 };
+
+// ---------------------------------------------------------------------------
+// Precision form (no counterpart in the reference): storage narrower than
+// the accumulation.  OGS_FLOAT_C stores float; OGS_BF16 stores bfloat16.
+// Add accumulates each row in float, in CSR order, with a two-sum running
+// error (the cascaded sum "Sum2" of Ogita, Rump and Oishi, SIAM J. Sci.
+// Comput. 26(6), 2005):
+//   s = +0, c = 0;  per member x:  t = s + x, z = t - s,
+//   e = (s - (t - z)) + (x - z)  (exact: s + x = t + e),  s = t, c += e;
+//   result = s + c when s is finite, else s (an overflow or inf/NaN member
+//   propagates as in a plain sum).
+// Its error is at most u|S| + gamma_{n-1}^2 sum|x| (u = 2^-24), i.e. the
+// float rounding of the exact row sum up to a term of order n^2 u^2.  bf16
+// results are then rounded to nearest even.  Min/Max compare-select the
+// (widened) stored values and are exact.  Mul is not offered for these types.
+struct f32c { float v; };       // OGS_FLOAT_C storage
+struct bf16s { uint16_t b; };   // OGS_BF16 storage
+
+struct Comp {
+  float s, c;
+};
+__device__ __forceinline__ void comp_add(Comp &a, float x) {
+  const float t = a.s + x;
+  const float z = t - a.s;
+  const float e = (a.s - (t - z)) + (x - z);
+  a.s = t;
+  a.c += e;
+}
+__device__ __forceinline__ float comp_fin(const Comp &a) { return __builtin_isfinite(a.s) ? a.s + a.c : a.s; }
+
+__device__ __forceinline__ float bf16_widen(bf16s x) { return __uint_as_float((uint32_t)x.b << 16); }
+// Round to nearest even; NaN stays NaN (quieted), overflow rounds to inf.
+__device__ __forceinline__ bf16s bf16_round(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16s{(uint16_t)((u >> 16) | 0x40u)};
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16s{(uint16_t)(u >> 16)};
+}
+constexpr float kBf16Max = 3.38953139e38f;  // 0x7f7f: largest finite bfloat16
+
+template <> struct Red<f32c, OGS_ADD> {
+  using A = Comp;
+  static __device__ __forceinline__ A init() { return Comp{0.0f, 0.0f}; }
+  static __device__ __forceinline__ void apply(A &a, f32c b) { comp_add(a, b.v); }
+  static __device__ __forceinline__ f32c fin(const A &a) { return f32c{comp_fin(a)}; }
+};
+template <> struct Red<f32c, OGS_MAX> {
+  using A = float;
+  static __device__ __forceinline__ A init() { return -FLT_MAX; }
+  static __device__ __forceinline__ void apply(A &a, f32c b) { a = (b.v > a) ? b.v : a; }
+  static __device__ __forceinline__ f32c fin(A a) { return f32c{a}; }
+};
+template <> struct Red<f32c, OGS_MIN> {
+  using A = float;
+  static __device__ __forceinline__ A init() { return FLT_MAX; }
+  static __device__ __forceinline__ void apply(A &a, f32c b) { a = (b.v < a) ? b.v : a; }
+  static __device__ __forceinline__ f32c fin(A a) { return f32c{a}; }
+};
+template <> struct Red<bf16s, OGS_ADD> {
+  using A = Comp;
+  static __device__ __forceinline__ A init() { return Comp{0.0f, 0.0f}; }
+  static __device__ __forceinline__ void apply(A &a, bf16s b) { comp_add(a, bf16_widen(b)); }
+  static __device__ __forceinline__ bf16s fin(const A &a) { return bf16_round(comp_fin(a)); }
+};
+template <> struct Red<bf16s, OGS_MAX> {
+  using A = float;
+  static __device__ __forceinline__ A init() { return -kBf16Max; }
+  static __device__ __forceinline__ void apply(A &a, bf16s b) {
+    const float x = bf16_widen(b);
+    a = (x > a) ? x : a;
+  }
+  static __device__ __forceinline__ bf16s fin(A a) { return bf16_round(a); }  // exact: a is a stored value
+};
+template <> struct Red<bf16s, OGS_MIN> {
+  using A = float;
+  static __device__ __forceinline__ A init() { return kBf16Max; }
+  static __device__ __forceinline__ void apply(A &a, bf16s b) {
+    const float x = bf16_widen(b);
+    a = (x < a) ? x : a;
+  }
+  static __device__ __forceinline__ bf16s fin(A a) { return bf16_round(a); }
+};
+
+// Whether (T, OP) has kernels: every op for the reference's types; Add,
+// Max and Min for the precision-form storage types.
+template <typename T, int OP>
+constexpr bool kOffered = !((std::is_same<T, f32c>::value || std::is_same<T, bf16s>::value) && OP == OGS_MUL);
 
 // Workgroup id -> block id such that XCD x (which receives workgroups
 // x, x+8, x+16, ...) processes one contiguous run of blocks.
@@ -156,11 +255,16 @@ __device__ __forceinline__ bool slot_words(const ListView &L, int b, bool full, 
 }
 
 // Column of a slot's index word.
+// (A two-level select on the segment bit pair: a flat compare chain over
+// bb's components can be turned into an indexed access of bb, which puts bb
+// in scratch memory.)
 __device__ __forceinline__ int slot_col(uint32_t w, const int4 &bb, bool packed) {
   if (!packed) return (int)w;
   const uint32_t sg = w >> kPackSegShift;
-  const int sb = sg == 0 ? bb.x : sg == 1 ? bb.y : sg == 2 ? bb.z : bb.w;
-  return sb + (int)((w >> kPackOffShift) & kPackOffMask);
+  const int b0 = bb.x, b1 = bb.y, b2 = bb.z, b3 = bb.w;
+  const int lo = (sg & 1u) ? b1 : b0;
+  const int hi = (sg & 1u) ? b3 : b2;
+  return ((sg & 2u) ? hi : lo) + (int)((w >> kPackOffShift) & kPackOffMask);
 }
 
 // LDS (CSR) position of slot j's entry.
@@ -187,86 +291,6 @@ __device__ __forceinline__ void stage_rows(const int (&v)[kRowSlots], int nr, in
   }
 }
 
-// GatherScatter, K = 1, gather list == scatter list (Sym, or an Unsigned
-// operator).  No row starts are read for a packed block: the pack words
-// carry a head mark at each row's first CSR position, and each row is reduced
-// from its head to the next head, in CSR order.  A block left in CSR order
-// marks its heads from the row starts.  Empty rows have nothing to gather or
-// write, so they need no mark.
-template <typename T, int OP>
-__device__ __forceinline__ void gs_sym_k1_block(const int4 *__restrict__ part, int b, int nrows, const ListView &G,
-                                                int gUseN, T *__restrict__ q, T *__restrict__ val,
-                                                uint8_t *__restrict__ head) {
-  Extent e;
-  if (!block_extent(part, b, nrows, G.starts, gUseN, G.starts, gUseN, e)) return;
-  const int n = e.nG;
-  if (n == 0) return;
-  const int tid = threadIdx.x;
-
-  uint32_t w[kSlots];
-  int4 gb;
-  int rg[kRowSlots];
-  const bool packed = slot_words(G, b, e.full, e.g0, n, w, gb);
-  if (!packed) row_words(G.starts, e.r0, e.nr, rg);
-  T x[kSlots];
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) x[j] = q[slot_col(w[j], gb, packed)];
-  if (packed) {
-#pragma unroll
-    for (int j = 0; j < kSlots; ++j) {
-      if (tid + j * kBlock < n) {
-        const int p = (int)(w[j] & kPackPosMask);
-        val[p] = x[j];
-        head[p] = (uint8_t)((w[j] >> kPackHeadBit) & 1u);
-      }
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < kSlots; ++j) {
-      const int i = tid + j * kBlock;
-      if (i < n) {
-        val[i] = x[j];
-        head[i] = 0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kRowSlots; ++j) {
-      const int r = tid + j * kBlock;
-      const int a = rg[j] - e.g0;
-      if (r < e.nr && a < n) head[a] = 1;  // an empty row marks the next row's head again
-    }
-  }
-  __syncthreads();
-
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int i = tid + j * kBlock;
-    if (i < n && head[i]) {
-      T acc = Red<T, OP>::init();
-      int k = i;
-      do {
-        Red<T, OP>::apply(acc, val[k]);
-        ++k;
-      } while (k < n && !head[k]);
-      for (int m = i; m < k; ++m) val[m] = acc;
-    }
-  }
-  __syncthreads();
-
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    if (tid + j * kBlock < n) q[slot_col(w[j], gb, packed)] = val[slot_pos(w[j], j, packed)];
-  }
-}
-
-template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_gs_sym_k1(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
-                                                      T *__restrict__ q) {
-  __shared__ T val[kCap];
-  __shared__ uint8_t head[kCap];
-  gs_sym_k1_block<T, OP>(part, xcd_block(blockIdx.x, gridDim.x), nrows, G, gUseN, q, val, head);
-}
 
 // GatherScatter Sym with KT interleaved fields (KT a compile-time count).
 // Lanes walk (entry, field) pairs, entries in the pack's column order, so a
@@ -348,12 +372,13 @@ __device__ __forceinline__ void gs_sym_kv_block(const int4 *__restrict__ part, i
   for (int t = tid; t < nv; t += kBlock) {
     const int i = fd.div(t), k = t - i * KF;
     if (head[i]) {
-      T acc = Red<T, OP>::init();
+      typename Red<T, OP>::A red = Red<T, OP>::init();
       int m = i;
       do {
-        Red<T, OP>::apply(acc, val[m * KF + k]);
+        Red<T, OP>::apply(red, val[m * KF + k]);
         ++m;
       } while (m < n && !head[m]);
+      const T acc = Red<T, OP>::fin(red);
       for (int r = i; r < m; ++r) val[r * KF + k] = acc;
     }
   }
@@ -369,16 +394,16 @@ __device__ __forceinline__ void gs_sym_kv_block(const int4 *__restrict__ part, i
   }
 }
 
-template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_gs_sym_kv(const int4 *__restrict__ part, int nrows, ListView G, int gUseN,
-                                                      T *__restrict__ q) {
-  __shared__ T val[kCap];
-  __shared__ uint8_t head[kCap];
-  gs_sym_kv_block<T, OP, KT>(part, xcd_block(blockIdx.x, gridDim.x), nrows, G, gUseN, q, val, head);
-}
-
 // Field counts with a compile-time kernel; other K use the field-pass kernel.
 __host__ __device__ constexpr bool packed_k(int K) { return K == 2 || K == 3 || K == 4 || K == 8 || K == 16; }
+
+// Entry-major layout of KT interleaved fields: a work item is VW
+// consecutive fields of one list entry (KT/VW items per entry), so one index
+// word feeds VW value loads from contiguous addresses.  A block stages at most
+// kVecBytes of values in LDS.
+constexpr int kVecBytes = 16384;
+template <typename T> __host__ __device__ constexpr int vec_cap() { return kVecBytes / (int)sizeof(T); }
+__host__ __device__ constexpr int vec_width(int KT) { return KT <= 4 ? KT : 4; }
 
 // GatherScatter, K = 1.  SYM: gather and scatter lists are the same array
 // (Sym, or an Unsigned operator): the scatter reuses the gather's slots.
@@ -418,8 +443,9 @@ __global__ __launch_bounds__(kBlock) void k_gs_k1(const int4 *__restrict__ part,
 
   for (int r = tid; r < e.nr; r += kBlock) {
     const int a = rsG[r], z = rsG[r + 1];
-    T acc = Red<T, OP>::init();
-    for (int i = a; i < z; ++i) Red<T, OP>::apply(acc, gval[i]);
+    typename Red<T, OP>::A red = Red<T, OP>::init();
+    for (int i = a; i < z; ++i) Red<T, OP>::apply(red, gval[i]);
+    const T acc = Red<T, OP>::fin(red);
     if constexpr (SYM) {
       for (int i = a; i < z; ++i) gval[i] = acc;
     } else {
@@ -467,8 +493,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_k1(const int4 *__restrict__ p
   __syncthreads();
   for (int r = tid; r < e.nr; r += kBlock) {
     const int a = rsG[r], z = rsG[r + 1];
-    T acc = Red<T, OP>::init();
-    for (int i = a; i < z; ++i) Red<T, OP>::apply(acc, gval[i]);
+    typename Red<T, OP>::A red = Red<T, OP>::init();
+    for (int i = a; i < z; ++i) Red<T, OP>::apply(red, gval[i]);
+    const T acc = Red<T, OP>::fin(red);
     gq[e.r0 + r] = acc;
   }
 }
@@ -492,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_k1(const int4 *__restrict__ 
 #pragma unroll
   for (int j = 0; j < kRowSlots; ++j) {
     const int r = tid + j * kBlock;
-    x[j] = r < e.nr ? gq[e.r0 + r] : T(0);
+    x[j] = r < e.nr ? gq[e.r0 + r] : T{};
   }
   stage_rows(rss, e.nr, e.s0, rsS);
   __syncthreads();
@@ -562,8 +589,9 @@ __device__ __forceinline__ void gs_kv_block(const int4 *__restrict__ part, int b
     for (int t = tid; t < e.nr * kw; t += kBlock) {
       const int r = t / kw, k = t - r * kw;
       const int a = rsG[r], b = rsG[r + 1];
-      T acc = Red<T, OP>::init();
-      for (int i = a; i < b; ++i) Red<T, OP>::apply(acc, gval[i * kw + k]);
+      typename Red<T, OP>::A red = Red<T, OP>::init();
+      for (int i = a; i < b; ++i) Red<T, OP>::apply(red, gval[i * kw + k]);
+      const T acc = Red<T, OP>::fin(red);
       if constexpr (SYM) {
         for (int i = a; i < b; ++i) gval[i * kw + k] = acc;
       } else {
@@ -681,6 +709,217 @@ __global__ __launch_bounds__(kBlock) void k_batch_gs_rest(const BatchDev *__rest
 }
 
 // ---------------------------------------------------------------------------
+// GatherScatter Sym (one list: gather list == scatter list) for KT = 1 and
+// KT in {2, 3, 4, 8, 16} interleaved fields, entry-major, one block per
+// workgroup.  A block goes through
+//   meta    its partition entries and pack base,
+//   index   its index words (pack words, or colIds in CSR order),
+//   gather  VW contiguous values per item, all slots' loads in flight,
+//   finish  LDS staging in CSR order, per-row reduction, scatter stores.
+// (Measured on MI355X: a persistent workgroup that walks a run of blocks and
+// loads block b+1's metadata and index words during block b's LDS work is
+// slower than one block per workgroup -- at 8 resident workgroups per CU
+// the hardware already overlaps the blocks' latencies, and its dispatcher
+// balances uneven blocks better than static runs.)
+//
+// Shape per (T, KT): SL slots per lane, sized so that the values in flight
+// stay within ~24 VGPRs; a block holds at most CAP entries (slots, LDS and the
+// pack's 2048 positions all bound it).
+template <typename T, int KT>
+struct VecShape {
+  static constexpr int VW = vec_width(KT);
+  static constexpr int C = KT / VW;
+  static constexpr int WORDS = (int)((sizeof(T) + 3) / 4);
+  static constexpr int SL = VW * WORDS <= 3 ? 8 : VW * WORDS <= 6 ? 4 : 2;
+  static constexpr int CAP0 = SL * kBlock / C < vec_cap<T>() / KT ? SL * kBlock / C : vec_cap<T>() / KT;
+  static constexpr int CAP = CAP0 < kCap ? CAP0 : kCap;
+  static_assert(KT % VW == 0, "field count must be a multiple of the item width");
+};
+
+// A block's list and buffer: fixed for a plan launch, per item in a batch.
+// These kernels run whole partitions (rows [0, NrowsT) of the T list, or of
+// an aliased N list), so no block is cut short at nrows.
+struct VecSrc {
+  const int4 *part;
+  const int32_t *ids;
+  const uint32_t *pk;
+  const int4 *base;
+  void *q;
+  int gUseN;
+};
+
+// A block's metadata: its partition entries and pack base.
+struct VecMeta {
+  int4 p0, p1, bb;
+};
+
+template <int SL>
+struct VecStage {
+  VecSrc src;
+  int n;       // entries
+  int g0;      // first entry
+  int4 bb;     // pack segment bases (x < 0: CSR order)
+  uint32_t w[SL];
+};
+
+__device__ __forceinline__ void vec_meta(const VecSrc &src, int b, VecMeta &m) {
+  m.p0 = src.part[b];
+  m.p1 = src.part[b + 1];
+  m.bb = src.pk ? src.base[b] : make_int4(-1, 0, 0, 0);
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Index words of the block described by m (its loads are issued here).
+template <typename T, int KT, int SL>
+__device__ __forceinline__ void vec_index(VecStage<SL> &s, const VecSrc &src, const VecMeta &m) {
+  constexpr int C = VecShape<T, KT>::C;
+  s.src = src;
+  s.g0 = uniform(src.gUseN ? m.p0.z : m.p0.y);
+  s.n = uniform(src.gUseN ? m.p1.z : m.p1.y) - s.g0;
+  s.bb = make_int4(uniform(m.bb.x), uniform(m.bb.y), uniform(m.bb.z), uniform(m.bb.w));
+  const uint32_t *w0 = s.bb.x >= 0 ? src.pk + s.g0 : reinterpret_cast<const uint32_t *>(src.ids) + s.g0;
+  const int ni = s.n * C;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = (int)threadIdx.x + j * kBlock;
+    s.w[j] = it < ni ? __builtin_nontemporal_load(w0 + it / C) : 0u;
+  }
+}
+
+// Gathers run unpredicated: a slot past the block's entries holds word 0,
+// which decodes to a valid column (see slot_words); the launcher runs these
+// kernels only on operators with at least one column.
+template <typename T, int KT, int SL>
+__device__ __forceinline__ void vec_gather(const VecStage<SL> &s, T (&x)[SL][VecShape<T, KT>::VW]) {
+  constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
+  const T *q = static_cast<const T *>(s.src.q);
+  const bool packed = s.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = (int)threadIdx.x + j * kBlock;
+    const T *a = q + (size_t)slot_col(s.w[j], s.bb, packed) * KT + (it % C) * VW;
+#pragma unroll
+    for (int v = 0; v < VW; ++v) x[j][v] = a[v];
+  }
+}
+
+template <typename T, int OP, int KT, int SL>
+__device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta &m,
+                                            const int32_t *__restrict__ starts,
+                                            const T (&x)[SL][VecShape<T, KT>::VW], T *__restrict__ val,
+                                            uint8_t *__restrict__ head) {
+  constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
+  const int n = s.n;
+  if (n == 0) return;
+  const int ni = n * C;
+  const int tid = threadIdx.x;
+  const bool packed = s.bb.x >= 0;
+  if (packed) {
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int it = tid + j * kBlock;
+      if (it < ni) {
+        const int c = it % C;
+        const int ps = (int)(s.w[j] & kPackPosMask);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = x[j][v];
+        if (c == 0) head[ps] = (uint8_t)((s.w[j] >> kPackHeadBit) & 1u);
+      }
+    }
+  } else {
+    // CSR order (a block the pack could not encode): heads from the row starts
+    const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;
+    int rg[kRowSlots];
+    row_words(starts, r0, nr, rg);
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int it = tid + j * kBlock;
+      if (it < ni) {
+        const int c = it % C, i = it / C;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) val[i * KT + c * VW + v] = x[j][v];
+        if (c == 0) head[i] = 0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRowSlots; ++j) {
+      const int r = tid + j * kBlock;
+      const int a = rg[j] - s.g0;
+      if (r < nr && a < n) head[a] = 1;  // an empty row marks the next row's head again
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < n * KT; t += kBlock) {
+    const int i = t / KT, k = t - i * KT;
+    if (head[i]) {
+      typename Red<T, OP>::A red = Red<T, OP>::init();
+      int mm = i;
+      do {
+        Red<T, OP>::apply(red, val[mm * KT + k]);
+        ++mm;
+      } while (mm < n && !head[mm]);
+      const T acc = Red<T, OP>::fin(red);
+      for (int r = i; r < mm; ++r) val[r * KT + k] = acc;
+    }
+  }
+  __syncthreads();
+  T *q = static_cast<T *>(s.src.q);
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < ni) {
+      const int c = it % C;
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
+      T *d = q + (size_t)slot_col(s.w[j], s.bb, packed) * KT + c * VW;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) d[v] = val[ps * KT + c * VW + v];
+    }
+  }
+}
+
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ part,
+                                                       const int32_t *__restrict__ starts,
+                                                       const int32_t *__restrict__ ids,
+                                                       const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int gUseN,
+                                                       T *__restrict__ q) {
+  using V = VecShape<T, KT>;
+  __shared__ T val[vec_cap<T>()];
+  __shared__ uint8_t head[kCap];
+  const VecSrc src = {part, ids, pk, base, q, gUseN};
+  VecStage<V::SL> s;
+  VecMeta m;
+  T x[V::SL][V::VW];
+  vec_meta(src, xcd_block(blockIdx.x, gridDim.x), m);
+  vec_index<T, KT>(s, src, m);
+  vec_gather<T, KT>(s, x);
+  vec_finish<T, OP, KT>(s, m, starts, x, val, head);
+}
+
+// Batched form: one (descriptor, block) item of one field-count class per
+// workgroup; the items of each XCD form one contiguous run of the list.
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict__ descs,
+                                                      const int2 *__restrict__ items) {
+  using V = VecShape<T, KT>;
+  __shared__ T val[vec_cap<T>()];
+  __shared__ uint8_t head[kCap];
+  const int2 w = items[xcd_block(blockIdx.x, gridDim.x)];
+  const BatchDev d = load_desc(descs, w.x);
+  const VecSrc src = {d.part, d.G.ids, d.G.pk, d.G.base, d.q, 0};
+  VecStage<V::SL> s;
+  VecMeta m;
+  T x[V::SL][V::VW];
+  vec_meta(src, w.y, m);
+  vec_index<T, KT>(s, src, m);
+  vec_gather<T, KT>(s, x);
+  vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head);
+}
+
+// ---------------------------------------------------------------------------
 // Gather: gv[k + row*K] = OP over the row's list.  Rows [0, nrows).
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_gather(const int4 *__restrict__ part, int nrows,
@@ -725,8 +964,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int4 *__restrict__ part
     for (int t = tid; t < e.nr * kw; t += kBlock) {
       const int r = t / kw, k = t - r * kw;
       const int a = rsG[r], b = rsG[r + 1];
-      T acc = Red<T, OP>::init();
-      for (int i = a; i < b; ++i) Red<T, OP>::apply(acc, gval[i * kw + k]);
+      typename Red<T, OP>::A red = Red<T, OP>::init();
+      for (int i = a; i < b; ++i) Red<T, OP>::apply(red, gval[i * kw + k]);
+      const T acc = Red<T, OP>::fin(red);
       gq[(size_t)(e.r0 + r) * K + k0 + k] = acc;
     }
     __syncthreads();
@@ -1037,6 +1277,79 @@ static ListView list_view(const Plan *p, bool listN, const Partition *pt, bool p
   return L;
 }
 
+// Field counts with a compile-time shape (k_gs_sym_vec, k_batch_vec).
+__host__ __device__ constexpr bool vec_k(int K) { return K == 1 || packed_k(K); }
+
+template <typename T>
+static int vec_cap_entries(int K) {
+  switch (K) {
+    case 1: return VecShape<T, 1>::CAP;
+    case 2: return VecShape<T, 2>::CAP;
+    case 3: return VecShape<T, 3>::CAP;
+    case 4: return VecShape<T, 4>::CAP;
+    case 8: return VecShape<T, 8>::CAP;
+    default: return VecShape<T, 16>::CAP;
+  }
+}
+
+// The widest T partition whose blocks hold at most the shape's entries, made
+// `narrow` steps narrower when there is one.
+template <typename T>
+static const Partition *vec_partition(const Plan *p, int K, int narrow) {
+  const int cap = vec_cap_entries<T>(K);
+  for (int j = 0; j < p->nparts; ++j) {
+    if (p->partT[j].maxT <= cap) {
+      const int jj = j + narrow < p->nparts ? j + narrow : p->nparts - 1;
+      return &p->partT[jj];
+    }
+  }
+  return nullptr;
+}
+
+template <typename T, int OP, int KT>
+static void launch_vec_k(const Partition *pt, const ListView &G, int gN, T *v, hipStream_t st) {
+  hipLaunchKernelGGL((k_gs_sym_vec<T, OP, KT>), dim3(pt->nblocks), dim3(kBlock), 0, st, pt->d, G.starts, G.ids, G.pk,
+                     G.base, gN, v);
+}
+
+template <typename T, int OP>
+static void launch_vec_op(int K, const Partition *pt, const ListView &G, int gN, T *v, hipStream_t st) {
+  switch (K) {
+    case 1: launch_vec_k<T, OP, 1>(pt, G, gN, v, st); break;
+    case 2: launch_vec_k<T, OP, 2>(pt, G, gN, v, st); break;
+    case 3: launch_vec_k<T, OP, 3>(pt, G, gN, v, st); break;
+    case 4: launch_vec_k<T, OP, 4>(pt, G, gN, v, st); break;
+    case 8: launch_vec_k<T, OP, 8>(pt, G, gN, v, st); break;
+    default: launch_vec_k<T, OP, 16>(pt, G, gN, v, st); break;
+  }
+}
+
+template <typename T>
+static int launch_vec(int op, int K, const Partition *pt, const ListView &G, int gN, T *v, hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_vec_op<T, OGS_ADD>(K, pt, G, gN, v, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_vec_op<T, OGS_MUL>(K, pt, G, gN, v, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_vec_op<T, OGS_MAX>(K, pt, G, gN, v, st); break;
+    case OGS_MIN: launch_vec_op<T, OGS_MIN>(K, pt, G, gN, v, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+// Resident workgroups per CU of a kernel (queried once per instance).
+template <typename F>
+static int occupancy_of(F kernel) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(kernel), kBlock, 0) !=
+          hipSuccess ||
+      n < 1)
+    n = 1;
+  return n;
+}
+
 template <typename T>
 static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:321-339
@@ -1061,25 +1374,26 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   const dim3 grid(pt->nblocks), block(kBlock);
   T *v = static_cast<T *>(d_v);
   const ListView G = list_view(p, gListN, pt, false), S = list_view(p, sListN, pt, false);
+  // one list (rows [0, NrowsT)) and a field count with a compile-time
+  // shape: the entry-major kernel on the widest partition whose blocks fit
+  // that shape.  For K = 1 partition 0 always fits.
+  if (sym && p->nnzT == 0) return OGS_SUCCESS;  // no entries: nothing to write
+  if (sym && vec_k(K)) {
+    const Partition *vp = vec_partition<T>(p, K, 0);
+    if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
+  }
 #define OGS_GS_LAUNCH(OPV)                                                                                      \
-  if (K == 1) {                                                                                                 \
-    if (sym) hipLaunchKernelGGL((k_gs_sym_k1<T, OPV>), grid, block, 0, st, pt->d, nrows, G, gN, v);              \
-    else hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);       \
-  } else if (sym && kc == K && packed_k(K)) {                                                                    \
-    switch (K) {                                                                                                \
-      case 2: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 2>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
-      case 3: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 3>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
-      case 4: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 4>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
-      case 8: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 8>), grid, block, 0, st, pt->d, nrows, G, gN, v); break;   \
-      default: hipLaunchKernelGGL((k_gs_sym_kv<T, OPV, 16>), grid, block, 0, st, pt->d, nrows, G, gN, v); break; \
-    }                                                                                                           \
+  if (K == 1 && !sym) {                                                                                         \
+    hipLaunchKernelGGL((k_gs_k1<T, OPV, false>), grid, block, 0, st, pt->d, nrows, G, gN, S, sN, v);            \
   } else {                                                                                                      \
     if (sym) hipLaunchKernelGGL((k_gs_kv<T, OPV, true>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v); \
     else hipLaunchKernelGGL((k_gs_kv<T, OPV, false>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, sS, sI, sN, K, kc, v);   \
   }
   switch (op) {
     case OGS_ADD: OGS_GS_LAUNCH(OGS_ADD) break;
-    case OGS_MUL: OGS_GS_LAUNCH(OGS_MUL) break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) { OGS_GS_LAUNCH(OGS_MUL) } else { return OGS_E_NOT_SUPPORTED; }
+      break;
     case OGS_MAX: OGS_GS_LAUNCH(OGS_MAX) break;
     case OGS_MIN: OGS_GS_LAUNCH(OGS_MIN) break;
     default: return OGS_E_ARG;
@@ -1111,7 +1425,9 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
   else hipLaunchKernelGGL((k_gather<T, OPV>), grid, block, 0, st, pt->d, nrows, gS, gI, gN, K, kc, v, gv);
   switch (op) {
     case OGS_ADD: OGS_G_LAUNCH(OGS_ADD) break;
-    case OGS_MUL: OGS_G_LAUNCH(OGS_MUL) break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) { OGS_G_LAUNCH(OGS_MUL) } else { return OGS_E_NOT_SUPPORTED; }
+      break;
     case OGS_MAX: OGS_G_LAUNCH(OGS_MAX) break;
     case OGS_MIN: OGS_G_LAUNCH(OGS_MIN) break;
     default: return OGS_E_ARG;
@@ -1154,7 +1470,9 @@ static int launch_extract_typed(int32_t N, int32_t K, const int32_t *ids, const 
   return (int)hipGetLastError();
 }
 
-static bool bad_type(int t) { return t < OGS_FLOAT || t > OGS_INT64; }
+static bool bad_type(int t) { return t < OGS_FLOAT || t > OGS_FLOAT_C; }
+// Mul is not offered on the precision-form storage types.
+static bool not_offered(int type, int op) { return (type == OGS_BF16 || type == OGS_FLOAT_C) && op == OGS_MUL; }
 static bool bad_op(int o) { return o < OGS_ADD || o > OGS_MIN; }
 static bool bad_trans(int t) { return t < OGS_SYM || t > OGS_TRANS; }
 
@@ -1285,7 +1603,9 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
   p->ciT = d_colIdsT;
   p->aliased = (d_rowStartsN == d_rowStartsT && d_colIdsN == d_colIdsT);
   int rc = (int)hipGetDevice(&p->device);
+  if (!rc) rc = (int)hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
   if (rc) { delete p; return rc; }
+  if (p->cus < 1) p->cus = 1;
 
   int32_t t0 = 0, tN = 0, n0 = 0, nN = 0;
   if ((rc = read_int(d_rowStartsT, st, &t0)) || (rc = read_int(d_rowStartsT + NrowsT, st, &tN)) ||
@@ -1362,25 +1682,31 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
 
 int ogs_hip_gather_scatter(ogs_hip_plan_t p, int type, int op, int trans, int32_t K, void *d_v, void *stream) {
   if (!p || bad_type(type) || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;
+  if (not_offered(type, op)) return OGS_E_NOT_SUPPORTED;
   if (!d_v && p->Ncols > 0) return OGS_E_ARG;
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (type) {
     case OGS_FLOAT: return launch_gs_typed<float>(p, op, trans, K, d_v, st);
     case OGS_DOUBLE: return launch_gs_typed<double>(p, op, trans, K, d_v, st);
     case OGS_INT32: return launch_gs_typed<int32_t>(p, op, trans, K, d_v, st);
-    default: return launch_gs_typed<int64_t>(p, op, trans, K, d_v, st);
+    case OGS_INT64: return launch_gs_typed<int64_t>(p, op, trans, K, d_v, st);
+    case OGS_BF16: return launch_gs_typed<bf16s>(p, op, trans, K, d_v, st);
+    default: return launch_gs_typed<f32c>(p, op, trans, K, d_v, st);
   }
 }
 
 int ogs_hip_gather(ogs_hip_plan_t p, int type, int op, int trans, int32_t K, void *d_gv, const void *d_v, void *stream) {
   if (!p || bad_type(type) || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;
+  if (not_offered(type, op)) return OGS_E_NOT_SUPPORTED;
   if ((!d_v && p->Ncols > 0) || (!d_gv && p->NrowsT > 0)) return OGS_E_ARG;
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (type) {
     case OGS_FLOAT: return launch_gather_typed<float>(p, op, trans, K, d_gv, d_v, st);
     case OGS_DOUBLE: return launch_gather_typed<double>(p, op, trans, K, d_gv, d_v, st);
     case OGS_INT32: return launch_gather_typed<int32_t>(p, op, trans, K, d_gv, d_v, st);
-    default: return launch_gather_typed<int64_t>(p, op, trans, K, d_gv, d_v, st);
+    case OGS_INT64: return launch_gather_typed<int64_t>(p, op, trans, K, d_gv, d_v, st);
+    case OGS_BF16: return launch_gather_typed<bf16s>(p, op, trans, K, d_gv, d_v, st);
+    default: return launch_gather_typed<f32c>(p, op, trans, K, d_gv, d_v, st);
   }
 }
 
@@ -1392,7 +1718,9 @@ int ogs_hip_scatter(ogs_hip_plan_t p, int type, int trans, int32_t K, void *d_v,
     case OGS_FLOAT: return launch_scatter_typed<float>(p, trans, K, d_v, d_gv, st);
     case OGS_DOUBLE: return launch_scatter_typed<double>(p, trans, K, d_v, d_gv, st);
     case OGS_INT32: return launch_scatter_typed<int32_t>(p, trans, K, d_v, d_gv, st);
-    default: return launch_scatter_typed<int64_t>(p, trans, K, d_v, d_gv, st);
+    case OGS_INT64: return launch_scatter_typed<int64_t>(p, trans, K, d_v, d_gv, st);
+    case OGS_BF16: return launch_scatter_typed<bf16s>(p, trans, K, d_v, d_gv, st);
+    default: return launch_scatter_typed<f32c>(p, trans, K, d_v, d_gv, st);
   }
 }
 
@@ -1404,7 +1732,9 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids, const 
     case OGS_FLOAT: return launch_extract_typed<float>(N, K, d_ids, d_q, d_gq, st);
     case OGS_DOUBLE: return launch_extract_typed<double>(N, K, d_ids, d_q, d_gq, st);
     case OGS_INT32: return launch_extract_typed<int32_t>(N, K, d_ids, d_q, d_gq, st);
-    default: return launch_extract_typed<int64_t>(N, K, d_ids, d_q, d_gq, st);
+    case OGS_INT64: return launch_extract_typed<int64_t>(N, K, d_ids, d_q, d_gq, st);
+    case OGS_BF16: return launch_extract_typed<bf16s>(N, K, d_ids, d_q, d_gq, st);
+    default: return launch_extract_typed<f32c>(N, K, d_ids, d_q, d_gq, st);
   }
 }
 
@@ -1416,23 +1746,34 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids, const 
 namespace ogs_hip {
 
 struct BatchConfig {
-  int grid;
   int narrow;  // K > 1 descriptors use the next narrower block width
-  int order;   // 0: largest items first, 1: descriptor order
+  int order;   // 0: items of larger descriptors first, 1: descriptor order
 };
+
+// Field-count classes with a compile-time shape: K = 1, 2, 3, 4, 8, 16.
+constexpr int kPipeClasses = 6;
+constexpr int kPipeK[kPipeClasses] = {1, 2, 3, 4, 8, 16};
+static int pipe_class(int K) {
+  for (int c = 0; c < kPipeClasses; ++c)
+    if (kPipeK[c] == K) return c;
+  return -1;
+}
 
 struct BatchTables {
   std::vector<BatchDev> descs;
-  std::vector<int2> items;  // K = 1 and one-pass K-field blocks (k_batch_gs)
-  std::vector<int2> rest;   // blocks of descriptors needing several field passes (k_batch_gs_rest)
+  std::vector<int2> cls[kPipeClasses];  // blocks of each shaped class (k_batch_vec<KT>)
+  std::vector<int2> items;              // other K, one pass (k_batch_gs)
+  std::vector<int2> rest;               // other K, several field passes (k_batch_gs_rest)
 };
 
+template <typename T>
 static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
                         const BatchConfig &c, BatchTables &t) {
   t.descs.clear();
+  for (auto &v : t.cls) v.clear();
   t.items.clear();
   t.rest.clear();
-  std::vector<int64_t> cost;
+  std::vector<const Partition *> parts(in.size(), nullptr);
   for (size_t i = 0; i < in.size(); ++i) {
     const Plan *p = in[i].plan;
     const int K = in[i].K;
@@ -1443,100 +1784,140 @@ static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::
     d.kmagic = K >= 2 ? (uint32_t)((uint64_t(1) << 32) / (uint64_t)K + 1u) : 0u;
     d.nrows = p->NrowsT;
     const Partition *pt = nullptr;
-    if (p->NrowsT > 0 && p->nparts > 0) {
-      int kc = 1;
-      if (K == 1) {
-        pt = &p->partT[0];
-      } else {
+    if (p->NrowsT > 0 && p->nparts > 0 && p->nnzT > 0) {  // no entries: nothing to reduce or write
+      int kc = K;
+      if (pipe_class(K) >= 0) pt = vec_partition<T>(p, K, K > 1 ? c.narrow : 0);
+      if (!pt) {
         pt = choose(p->partT, p->nparts, K, [](const Partition &x) { return x.maxT; }, &kc);
         if (!pt) return OGS_E_ROW_TOO_LONG;
-        const int j = (int)(pt - p->partT);
-        if (c.narrow && kc == K && j + 1 < p->nparts) pt = &p->partT[j + 1];
       }
       d.kc = kc;
       d.part = pt->d;
       d.G = list_view(p, false, pt, false);
     }
     t.descs.push_back(d);
-    if (pt) {
-      for (int b = 0; b < pt->nblocks; ++b) {
-        if (d.kc == K) {
-          t.items.push_back(make_int2((int)i, b));
-          cost.push_back((int64_t)pt->W * K);
-        } else {
-          t.rest.push_back(make_int2((int)i, b));
-        }
-      }
-    }
+    parts[i] = pt;
   }
+  // descriptor order of the item lists: larger work first, or as given
+  std::vector<size_t> ord(in.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
   if (c.order == 0) {
-    std::vector<size_t> idx(t.items.size());
-    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    std::vector<int2> sorted(idx.size());
-    for (size_t i = 0; i < idx.size(); ++i) sorted[i] = t.items[idx[i]];
-    t.items.swap(sorted);
+    auto cost = [&](size_t i) { return (int64_t)in[i].plan->nnzT * in[i].K; };
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+  }
+  for (size_t i : ord) {
+    const Partition *pt = parts[i];
+    if (!pt) continue;
+    const int K = in[i].K, cl = pipe_class(K);
+    const bool shaped = cl >= 0 && t.descs[i].kc == K && pt->maxT <= vec_cap_entries<T>(K);
+    std::vector<int2> &dst = shaped ? t.cls[cl] : t.descs[i].kc == K ? t.items : t.rest;
+    for (int b = 0; b < pt->nblocks; ++b) dst.push_back(make_int2((int)i, b));
   }
   return OGS_SUCCESS;
-}
-
-template <typename T>
-static int launch_batch_typed(int op, const BatchDev *d, const int2 *items, int n, const int2 *rest, int nrest,
-                              int grid, hipStream_t st) {
-#define OGS_B_LAUNCH(OPV)                                                                                         \
-  if (n) hipLaunchKernelGGL((k_batch_gs<T, OPV>), dim3(grid), dim3(kBlock), 0, st, d, items, n);                  \
-  if (nrest) hipLaunchKernelGGL((k_batch_gs_rest<T, OPV>), dim3(grid), dim3(kBlock), 0, st, d, rest, nrest);
-  switch (op) {
-    case OGS_ADD: OGS_B_LAUNCH(OGS_ADD) break;
-    case OGS_MUL: OGS_B_LAUNCH(OGS_MUL) break;
-    case OGS_MAX: OGS_B_LAUNCH(OGS_MAX) break;
-    case OGS_MIN: OGS_B_LAUNCH(OGS_MIN) break;
-    default: return OGS_E_ARG;
-  }
-#undef OGS_B_LAUNCH
-  return (int)hipGetLastError();
 }
 
 // Device copies of a batch's tables.
 struct BatchDevTables {
   BatchDev *d = nullptr;
-  int2 *items = nullptr;
-  int2 *rest = nullptr;
-  int n = 0, nrest = 0;
+  int2 *lists = nullptr;  // the class lists, then items, then rest
+  int n[kPipeClasses + 2] = {};
+  size_t off[kPipeClasses + 2] = {};
   void release() {
     if (d) (void)hipFree(d);
-    if (items) (void)hipFree(items);
-    if (rest) (void)hipFree(rest);
+    if (lists) (void)hipFree(lists);
     d = nullptr;
-    items = rest = nullptr;
-    n = nrest = 0;
+    lists = nullptr;
+    for (int c = 0; c < kPipeClasses + 2; ++c) n[c] = 0, off[c] = 0;
+  }
+  int total() const {
+    int s = 0;
+    for (int c = 0; c < kPipeClasses + 2; ++c) s += n[c];
+    return s;
   }
 };
 
-static int launch_batch(int type, int op, const BatchDevTables &t, int grid, hipStream_t st) {
+template <typename T, int OP, int KT>
+static void launch_batch_class(const BatchDev *d, const int2 *items, int n, hipStream_t st) {
+  hipLaunchKernelGGL((k_batch_vec<T, OP, KT>), dim3(n), dim3(kBlock), 0, st, d, items);
+}
+
+template <typename T, int OP>
+static void launch_batch_op(const BatchDevTables &t, int cus, hipStream_t st) {
+  const int2 *L = t.lists;
+  if (t.n[0]) launch_batch_class<T, OP, 1>(t.d, L + t.off[0], t.n[0], st);
+  if (t.n[1]) launch_batch_class<T, OP, 2>(t.d, L + t.off[1], t.n[1], st);
+  if (t.n[2]) launch_batch_class<T, OP, 3>(t.d, L + t.off[2], t.n[2], st);
+  if (t.n[3]) launch_batch_class<T, OP, 4>(t.d, L + t.off[3], t.n[3], st);
+  if (t.n[4]) launch_batch_class<T, OP, 8>(t.d, L + t.off[4], t.n[4], st);
+  if (t.n[5]) launch_batch_class<T, OP, 16>(t.d, L + t.off[5], t.n[5], st);
+  // other K: persistent grid-stride kernels over their items
+  static const int occ = occupancy_of(k_batch_gs<T, OP>);
+  const int g = occ * cus;
+  const int n6 = t.n[kPipeClasses], n7 = t.n[kPipeClasses + 1];
+  if (n6) hipLaunchKernelGGL((k_batch_gs<T, OP>), dim3(g < n6 ? g : n6), dim3(kBlock), 0, st, t.d,
+                             L + t.off[kPipeClasses], n6);
+  if (n7) hipLaunchKernelGGL((k_batch_gs_rest<T, OP>), dim3(g < n7 ? g : n7), dim3(kBlock), 0, st, t.d,
+                             L + t.off[kPipeClasses + 1], n7);
+}
+
+template <typename T>
+static int launch_batch_typed(int op, const BatchDevTables &t, int cus, hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_batch_op<T, OGS_ADD>(t, cus, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_batch_op<T, OGS_MUL>(t, cus, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_batch_op<T, OGS_MAX>(t, cus, st); break;
+    case OGS_MIN: launch_batch_op<T, OGS_MIN>(t, cus, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename F>
+static int by_storage(int type, F &&f) {
   switch (type) {
-    case OGS_FLOAT: return launch_batch_typed<float>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
-    case OGS_DOUBLE: return launch_batch_typed<double>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
-    case OGS_INT32: return launch_batch_typed<int32_t>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
-    case OGS_INT64: return launch_batch_typed<int64_t>(op, t.d, t.items, t.n, t.rest, t.nrest, grid, st);
+    case OGS_FLOAT: return f(float{});
+    case OGS_DOUBLE: return f(double{});
+    case OGS_INT32: return f(int32_t{});
+    case OGS_INT64: return f(int64_t{});
+    case OGS_BF16: return f(bf16s{});
+    case OGS_FLOAT_C: return f(f32c{});
     default: return OGS_E_ARG;
   }
 }
 
+static int launch_batch(int type, int op, const BatchDevTables &t, int cus, hipStream_t st) {
+  return by_storage(type, [&](auto tag) { return launch_batch_typed<decltype(tag)>(op, t, cus, st); });
+}
+
+static int make_tables(int type, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
+                       const BatchConfig &c, BatchTables &t) {
+  return by_storage(type, [&](auto tag) { return batch_tables<decltype(tag)>(in, qs, c, t); });
+}
+
 static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &out) {
   out.release();
+  const std::vector<int2> *src[kPipeClasses + 2];
+  for (int c = 0; c < kPipeClasses; ++c) src[c] = &t.cls[c];
+  src[kPipeClasses] = &t.items;
+  src[kPipeClasses + 1] = &t.rest;
+  size_t total = 0;
+  for (int c = 0; c < kPipeClasses + 2; ++c) {
+    out.off[c] = total;
+    out.n[c] = (int)src[c]->size();
+    total += src[c]->size();
+  }
   hipError_t e = hipMalloc(&out.d, sizeof(BatchDev) * std::max<size_t>(t.descs.size(), 1));
-  if (e == hipSuccess) e = hipMalloc(&out.items, sizeof(int2) * std::max<size_t>(t.items.size(), 1));
+  if (e == hipSuccess) e = hipMalloc(&out.lists, sizeof(int2) * std::max<size_t>(total, 1));
   if (e == hipSuccess && !t.descs.empty())
     e = hipMemcpyAsync(out.d, t.descs.data(), sizeof(BatchDev) * t.descs.size(), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMalloc(&out.rest, sizeof(int2) * std::max<size_t>(t.rest.size(), 1));
-  if (e == hipSuccess && !t.items.empty())
-    e = hipMemcpyAsync(out.items, t.items.data(), sizeof(int2) * t.items.size(), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && !t.rest.empty())
-    e = hipMemcpyAsync(out.rest, t.rest.data(), sizeof(int2) * t.rest.size(), hipMemcpyHostToDevice, st);
+  for (int c = 0; c < kPipeClasses + 2 && e == hipSuccess; ++c)
+    if (out.n[c])
+      e = hipMemcpyAsync(out.lists + out.off[c], src[c]->data(), sizeof(int2) * src[c]->size(),
+                         hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  out.n = (int)t.items.size();
-  out.nrest = (int)t.rest.size();
   return (int)e;
 }
 
@@ -1545,7 +1926,8 @@ static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &o
 struct ogs_hip_batch_s {
   int type = 0, op = 0;
   ogs_hip::BatchDevTables tables;
-  ogs_hip::BatchConfig cfg = {0, 0, 0};
+  ogs_hip::BatchConfig cfg = {0, 1};
+  int cus = 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int candidates = 0;
@@ -1555,7 +1937,7 @@ struct ogs_hip_batch_s {
 namespace ogs_hip {
 
 static size_t type_size(int type) {
-  return type == OGS_FLOAT || type == OGS_INT32 ? 4 : 8;
+  return type == OGS_BF16 ? 2 : type == OGS_FLOAT || type == OGS_INT32 || type == OGS_FLOAT_C ? 4 : 8;
 }
 
 static void free_batch(ogs_hip_batch_s *b) {
@@ -1567,18 +1949,18 @@ static void free_batch(ogs_hip_batch_s *b) {
 
 // Times a configuration on scratch buffers: 1 warm-up, 3 timed launches.
 static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
-                       const BatchConfig &c, hipStream_t st, float *us) {
+                       const BatchConfig &c, int cus, hipStream_t st, float *us) {
   BatchTables t;
-  int rc = batch_tables(in, qs, c, t);
+  int rc = make_tables(type, in, qs, c, t);
   if (rc) return rc;
   BatchDevTables dt;
   rc = upload_tables(t, st, dt);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
-  if (!rc) rc = launch_batch(type, op, dt, c.grid, st);
+  if (!rc) rc = launch_batch(type, op, dt, cus, st);
   if (!rc) rc = (int)hipEventRecord(e0, st);
-  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt, c.grid, st);
+  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt, cus, st);
   if (!rc) rc = (int)hipEventRecord(e1, st);
   if (!rc) rc = (int)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -1600,6 +1982,7 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!batch) return OGS_E_ARG;
   *batch = nullptr;
   if (bad_type(type) || bad_op(op) || n < 0 || (n > 0 && !descs)) return OGS_E_ARG;
+  if (not_offered(type, op)) return OGS_E_NOT_SUPPORTED;
   std::vector<ogs_hip_batch_desc_t> in(descs, descs + n);
   std::vector<void *> qs(n);
   bool anyKv = false;
@@ -1616,7 +1999,9 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   int dev = 0, cus = 256;
   int rc = (int)hipGetDevice(&dev);
   if (!rc) rc = (int)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  BatchConfig best = {cus * 6, 0, 0};
+  if (cus < 1) cus = 1;
+  b->cus = cus;
+  BatchConfig best = {0, 1};
   const auto t0 = std::chrono::steady_clock::now();
   if (!rc && tune && n > 0) {
     // scratch copies: tuning launches must not touch the caller's data
@@ -1628,13 +2013,13 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
       if (!rc) rc = (int)hipMemcpyAsync(scratch[i], in[i].d_v, bytes, hipMemcpyDeviceToDevice, st);
     }
     float bestUs = 0.f;
-    for (int g : {cus * 4, cus * 6, cus * 8}) {
+    {
       for (int narrow = 0; narrow <= (anyKv ? 1 : 0); ++narrow) {
         for (int order = 0; order <= 1; ++order) {
           if (rc) break;
-          const BatchConfig c = {g, narrow, order};
+          const BatchConfig c = {narrow, order};
           float us = 0.f;
-          rc = time_config(type, op, in, scratch, c, st, &us);
+          rc = time_config(type, op, in, scratch, c, cus, st, &us);
           ++b->candidates;
           if (!rc && (bestUs == 0.f || us < bestUs)) {
             bestUs = us;
@@ -1650,14 +2035,14 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   b->tune_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   b->cfg = best;
   BatchTables t;
-  if (!rc) rc = batch_tables(in, qs, best, t);
+  if (!rc) rc = make_tables(type, in, qs, best, t);
   if (!rc) rc = upload_tables(t, st, b->tables);
   // capture the launch in a graph (capture needs a stream of its own)
   hipStream_t cs = nullptr;
   if (!rc) rc = (int)hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (!rc) rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (!rc) {
-    const int lr = launch_batch(type, op, b->tables, best.grid, cs);
+    const int lr = launch_batch(type, op, b->tables, cus, cs);
     const int er = (int)hipStreamEndCapture(cs, &b->graph);
     rc = lr ? lr : er;
   }
@@ -1684,8 +2069,10 @@ int ogs_hip_batch_destroy(ogs_hip_batch_t batch) {
 
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info) {
   if (!batch || !info) return OGS_E_ARG;
-  info->items = batch->tables.n + batch->tables.nrest;
-  info->grid = batch->cfg.grid;
+  info->items = batch->tables.total();
+  int nodes = 0;
+  for (int c = 0; c < ogs_hip::kPipeClasses + 2; ++c) nodes += batch->tables.n[c] > 0;
+  info->nodes = nodes;
   info->narrow = batch->cfg.narrow;
   info->order = batch->cfg.order;
   info->candidates = batch->candidates;

@@ -17,21 +17,29 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 
-# enums: include/ogs.hpp:179-200 (+ bf16 storage for the precision form)
-Float, Double, Int32, Int64, BF16 = 0, 1, 2, 3, 4
+# enums: include/ogs.hpp:179-200, plus the precision-form storage types:
+# BF16 (bfloat16 storage) and FloatC (float storage), both reducing Add in
+# float with two-sum compensation (ogs_type_t in include/ogs_hip.h)
+Float, Double, Int32, Int64, BF16, FloatC = 0, 1, 2, 3, 4, 5
 Add, Mul, Max, Min = 0, 1, 2, 3
 Sym, NoTrans, Trans = 0, 1, 2
 Auto, Pairwise, CrystalRouter, AllToAll = 0, 1, 2, 3
 Unsigned, Signed, Halo = 0, 1, 2
 
 
-def type_of(t) -> int:
+def type_of(t, type=None) -> int:
+    """Type code of a buffer: the one its dtype names, or `type` when given
+    (FloatC selects the compensated Add on a float32 buffer)."""
     import torch
     m = {torch.float32: Float, torch.float64: Double, torch.int32: Int32, torch.int64: Int64,
          torch.bfloat16: BF16}
     if t.dtype not in m:
         raise TypeError(f"unsupported dtype {t.dtype}")
-    return m[t.dtype]
+    if type is None:
+        return m[t.dtype]
+    if not (type == m[t.dtype] or (type == FloatC and t.dtype == torch.float32)):
+        raise TypeError(f"type {type} does not store {t.dtype}")
+    return type
 
 
 def _dev_ptr(t) -> int:
@@ -77,16 +85,16 @@ class Plan:
         check(lib().ogs_hip_plan_info(self.h, C.byref(i)), "ogs_hip_plan_info")
         return {n: getattr(i, n) for n, _ in i._fields_}
 
-    def GatherScatter(self, v, k, op, trans, stream=None):
-        check(lib().ogs_hip_gather_scatter(self.h, type_of(v), op, trans, k, _dev_ptr(v), _stream(stream)),
+    def GatherScatter(self, v, k, op, trans, stream=None, type=None):
+        check(lib().ogs_hip_gather_scatter(self.h, type_of(v, type), op, trans, k, _dev_ptr(v), _stream(stream)),
               "ogs_hip_gather_scatter")
 
-    def Gather(self, gv, v, k, op, trans, stream=None):
-        check(lib().ogs_hip_gather(self.h, type_of(v), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
+    def Gather(self, gv, v, k, op, trans, stream=None, type=None):
+        check(lib().ogs_hip_gather(self.h, type_of(v, type), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
               "ogs_hip_gather")
 
-    def Scatter(self, v, gv, k, trans, stream=None):
-        check(lib().ogs_hip_scatter(self.h, type_of(v), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
+    def Scatter(self, v, gv, k, trans, stream=None, type=None):
+        check(lib().ogs_hip_scatter(self.h, type_of(v, type), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
               "ogs_hip_scatter")
 
     def __del__(self):
@@ -103,15 +111,15 @@ class Batch:
     creation; tune=True times launch shapes on scratch copies first.  The
     plans' owners (ogs_t objects) must outlive the batch."""
 
-    def __init__(self, items, op: int = Add, tune: bool = True, stream=None):
+    def __init__(self, items, op: int = Add, tune: bool = True, stream=None, type=None):
         items = list(items)
         if not items:
             raise ValueError("empty batch")
-        ty = type_of(items[0][2])
+        ty = type_of(items[0][2], type)
         descs = (_lib.BatchDesc * len(items))()
         self._keep = []
         for i, (plan, K, v) in enumerate(items):
-            if type_of(v) != ty:
+            if type_of(v, type) != ty:
                 raise TypeError("all buffers of a batch share one dtype")
             if v.numel() < plan.info()["Ncols"] * K:
                 raise ValueError("buffer shorter than Ncols*K")
@@ -200,14 +208,14 @@ class ogs_t:
               "ogs_get_csr")
         return a
 
-    def GatherScatter(self, v, k: int, op: int, trans: int, stream=None) -> None:
-        check(lib().ogs_gather_scatter(self.h, type_of(v), op, trans, k, _dev_ptr(v), _stream(stream)),
+    def GatherScatter(self, v, k: int, op: int, trans: int, stream=None, type=None) -> None:
+        check(lib().ogs_gather_scatter(self.h, type_of(v, type), op, trans, k, _dev_ptr(v), _stream(stream)),
               "ogs_gather_scatter")
 
-    def Gather(self, gv, v, k: int, op: int, trans: int, stream=None) -> None:
-        check(lib().ogs_gather(self.h, type_of(v), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
+    def Gather(self, gv, v, k: int, op: int, trans: int, stream=None, type=None) -> None:
+        check(lib().ogs_gather(self.h, type_of(v, type), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
               "ogs_gather")
 
-    def Scatter(self, v, gv, k: int, trans: int, stream=None) -> None:
-        check(lib().ogs_scatter(self.h, type_of(v), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
+    def Scatter(self, v, gv, k: int, trans: int, stream=None, type=None) -> None:
+        check(lib().ogs_scatter(self.h, type_of(v, type), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
               "ogs_scatter")

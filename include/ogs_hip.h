@@ -37,9 +37,18 @@
 extern "C" {
 #endif
 
-/* Value types.  0..3 follow ogs::Type (include/ogs.hpp:179); OGS_BF16 is
- * bfloat16 storage with float accumulation (precision form only). */
-typedef enum { OGS_FLOAT = 0, OGS_DOUBLE = 1, OGS_INT32 = 2, OGS_INT64 = 3, OGS_BF16 = 4 } ogs_type_t;
+/* Value types.  0..3 follow ogs::Type (include/ogs.hpp:179) and reduce
+ * exactly as the reference's host operator does.  4 and 5 are the precision
+ * form (no counterpart in the reference): narrow storage, Add accumulated per
+ * row in float, in CSR order, with a two-sum running error (the cascaded sum
+ * "Sum2" of Ogita, Rump and Oishi, 2005), written back rounded to nearest
+ * (even) in the storage type; Max/Min exact; Mul -> OGS_E_NOT_SUPPORTED.
+ *   OGS_BF16     bfloat16 storage (2 bytes per value)
+ *   OGS_FLOAT_C  float storage (4 bytes), compensated Add; OGS_FLOAT keeps
+ *                the reference's plain float sum. */
+typedef enum {
+  OGS_FLOAT = 0, OGS_DOUBLE = 1, OGS_INT32 = 2, OGS_INT64 = 3, OGS_BF16 = 4, OGS_FLOAT_C = 5
+} ogs_type_t;
 /* ogs::Op (include/ogs.hpp:191) */
 typedef enum { OGS_ADD = 0, OGS_MUL = 1, OGS_MAX = 2, OGS_MIN = 3 } ogs_op_t;
 /* ogs::Transpose (include/ogs.hpp:194) */
@@ -116,12 +125,15 @@ typedef struct {
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
 
 /* Batched GatherScatter: many plans (meshes), each with its own K
- * interleaved fields, in one persistent launch captured in a HIP graph.
- * No counterpart in the reference (it launches one kernel per operator and
- * field set).  Every descriptor runs GatherScatter Sym with the batch's
- * type and op on its buffer d_v (Ncols*K values).  Buffers are bound at
- * creation.  With tune != 0 the creation times candidate launch shapes
- * (grid size, block width for K > 1, work order) on scratch copies of the
+ * interleaved fields, in one HIP graph launch.  No counterpart in the
+ * reference (it launches one kernel per operator and field set).  Every
+ * descriptor runs GatherScatter Sym with the batch's type and op on its
+ * buffer d_v (Ncols*K values); buffers must not overlap.  The graph holds
+ * one kernel node per field-count class present (K = 1, 2, 3, 4, 8, 16, each
+ * with its compiled shape, one workgroup per (descriptor, block) item) and a
+ * persistent grid-stride node for any other K.  Buffers are bound at
+ * creation.  With tune != 0 the creation times candidate configurations
+ * (block width of K > 1 classes, item order) on scratch copies of the
  * buffers and keeps the fastest; the buffers themselves are not touched.
  * Creation allocates and synchronises; ogs_hip_batch_run only launches the
  * graph.  Concurrent runs of one batch race on its buffers. */
@@ -135,11 +147,12 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
                          int32_t n, int tune, void *stream);
 int ogs_hip_batch_run(ogs_hip_batch_t batch, void *stream);
 int ogs_hip_batch_destroy(ogs_hip_batch_t batch);
-/* What the creation chose: work items, persistent grid, whether K>1
- * descriptors use the next narrower block width, work order (0 = largest
- * items first, 1 = descriptor order), and the tuning time / best run. */
+/* What the creation chose: work items (blocks over all descriptors),
+ * kernel nodes in the graph, whether K>1 classes use the next narrower
+ * block width, item order (0 = larger descriptors first, 1 = descriptor
+ * order), and the tuning time / best run. */
 typedef struct {
-  int32_t items, grid, narrow, order, candidates;
+  int32_t items, nodes, narrow, order, candidates;
   float tune_ms, best_us;
 } ogs_hip_batch_info_t;
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info);

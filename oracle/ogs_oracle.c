@@ -29,6 +29,7 @@
 #include <float.h>
 #include <limits.h>
 #include <stdlib.h>
+#include <math.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -435,35 +436,168 @@ DEF_ALL(f64, double, APPLY_ADD_F, APPLY_MUL_F, DBL_MAX)
 DEF_ALL(i32, int32_t, APPLY_ADD_I32, APPLY_MUL_I32, INT32_MAX)
 DEF_ALL(i64, int64_t, APPLY_ADD_I64, APPLY_MUL_I64, INT64_MAX)
 
+/* ------------------------------------------------------------------ */
+/* Precision form: OR_BF16 (bfloat16 storage) and OR_FLOAT_C (float
+ * storage).  No counterpart in the reference: this restates the library's
+ * own specification.  Add reduces each row in float, in CSR order, with a
+ * two-sum running error (cascaded sum "Sum2", Ogita, Rump and Oishi, SIAM J.
+ * Sci. Comput. 26(6), 2005): s = +0, c = 0; per member x: t = s + x,
+ * z = t - s, e = (s - (t - z)) + (x - z), s = t, c += e; result = s + c when
+ * s is finite, else s.  bf16 results round to nearest even (NaN quieted).
+ * Max/Min compare-select the widened stored values.  Mul: not offered. */
+typedef struct { float s, c; } comp_t;
+
+static inline void comp_add(comp_t *a, float x) {
+  const float t = a->s + x;
+  const float z = t - a->s;
+  const float e = (a->s - (t - z)) + (x - z);
+  a->s = t;
+  a->c += e;
+}
+static inline float comp_fin(comp_t a) { return isfinite(a.s) ? a.s + a.c : a.s; }
+
+static inline float bf16_widen(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static inline uint16_t bf16_round(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+#define BF16_MAXF 3.38953139e38f /* 0x7f7f */
+
+#define LOAD_F32(x) (x)
+#define STORE_F32(f) (f)
+#define LOAD_BF16(x) bf16_widen(x)
+#define STORE_BF16(f) bf16_round(f)
+
+#define DEF_GATHER_C(NAME, T, LOAD, STORE)                                             \
+  static void NAME(dlong Nrows, const dlong *rowStarts, const dlong *colIds, int K,  \
+                   const T *v, T *gv) {                                               \
+    _Pragma("omp parallel for")                                                       \
+    for (dlong n = 0; n < Nrows; ++n) {                                               \
+      const dlong start = rowStarts[n], end = rowStarts[n + 1];                       \
+      for (int k = 0; k < K; ++k) {                                                   \
+        comp_t a = {0.0f, 0.0f};                                                      \
+        for (dlong g = start; g < end; ++g) comp_add(&a, LOAD(v[k + (size_t)colIds[g] * K])); \
+        gv[k + (size_t)n * K] = STORE(comp_fin(a));                                   \
+      }                                                                               \
+    }                                                                                 \
+  }
+#define DEF_GS_C(NAME, T, LOAD, STORE)                                                 \
+  static void NAME(dlong Nrows, const dlong *gS, const dlong *gI, const dlong *sS,   \
+                   const dlong *sI, int K, T *v) {                                    \
+    _Pragma("omp parallel for")                                                       \
+    for (dlong n = 0; n < Nrows; ++n) {                                               \
+      for (int k = 0; k < K; ++k) {                                                   \
+        comp_t a = {0.0f, 0.0f};                                                      \
+        for (dlong g = gS[n]; g < gS[n + 1]; ++g) comp_add(&a, LOAD(v[k + (size_t)gI[g] * K])); \
+        const T val = STORE(comp_fin(a));                                             \
+        for (dlong s = sS[n]; s < sS[n + 1]; ++s) v[k + (size_t)sI[s] * K] = val;      \
+      }                                                                               \
+    }                                                                                 \
+  }
+/* bf16 Max/Min: compare-select in float, exact on the stored values */
+#define DEF_GATHER_MM16(NAME, INIT, CMP)                                               \
+  static void NAME(dlong Nrows, const dlong *rowStarts, const dlong *colIds, int K,  \
+                   const uint16_t *v, uint16_t *gv) {                                 \
+    _Pragma("omp parallel for")                                                       \
+    for (dlong n = 0; n < Nrows; ++n) {                                               \
+      for (int k = 0; k < K; ++k) {                                                   \
+        float a = INIT;                                                               \
+        for (dlong g = rowStarts[n]; g < rowStarts[n + 1]; ++g) {                     \
+          const float x = bf16_widen(v[k + (size_t)colIds[g] * K]);                   \
+          a = (x CMP a) ? x : a;                                                      \
+        }                                                                             \
+        gv[k + (size_t)n * K] = bf16_round(a);                                        \
+      }                                                                               \
+    }                                                                                 \
+  }
+#define DEF_GS_MM16(NAME, INIT, CMP)                                                   \
+  static void NAME(dlong Nrows, const dlong *gS, const dlong *gI, const dlong *sS,   \
+                   const dlong *sI, int K, uint16_t *v) {                             \
+    _Pragma("omp parallel for")                                                       \
+    for (dlong n = 0; n < Nrows; ++n) {                                               \
+      for (int k = 0; k < K; ++k) {                                                   \
+        float a = INIT;                                                               \
+        for (dlong g = gS[n]; g < gS[n + 1]; ++g) {                                   \
+          const float x = bf16_widen(v[k + (size_t)gI[g] * K]);                       \
+          a = (x CMP a) ? x : a;                                                      \
+        }                                                                             \
+        const uint16_t val = bf16_round(a);                                           \
+        for (dlong s = sS[n]; s < sS[n + 1]; ++s) v[k + (size_t)sI[s] * K] = val;      \
+      }                                                                               \
+    }                                                                                 \
+  }
+
+DEF_GATHER_C(gather_f32c_add, float, LOAD_F32, STORE_F32)
+DEF_GS_C(gs_f32c_add, float, LOAD_F32, STORE_F32)
+DEF_GATHER_C(gather_bf16_add, uint16_t, LOAD_BF16, STORE_BF16)
+DEF_GS_C(gs_bf16_add, uint16_t, LOAD_BF16, STORE_BF16)
+DEF_GATHER_MM16(gather_bf16_max, -BF16_MAXF, >)
+DEF_GATHER_MM16(gather_bf16_min, BF16_MAXF, <)
+DEF_GS_MM16(gs_bf16_max, -BF16_MAXF, >)
+DEF_GS_MM16(gs_bf16_min, BF16_MAXF, <)
+
+static void scatter_u16(dlong Nrows, const dlong *rowStarts, const dlong *colIds, int K, const uint16_t *gv,
+                        uint16_t *v) {
+#pragma omp parallel for
+  for (dlong n = 0; n < Nrows; ++n)
+    for (dlong g = rowStarts[n]; g < rowStarts[n + 1]; ++g)
+      for (int k = 0; k < K; ++k) v[k + (size_t)colIds[g] * K] = gv[k + (size_t)n * K];
+}
+static void extract_u16(dlong N, int K, const dlong *ids, const uint16_t *q, uint16_t *gq) {
+  for (dlong n = 0; n < N; ++n)
+    for (int k = 0; k < K; ++k) gq[k + (size_t)n * K] = q[k + (size_t)ids[n] * K];
+}
+
+float oracle_bf16_to_float(uint16_t b) { return bf16_widen(b); }
+uint16_t oracle_float_to_bf16(float f) { return bf16_round(f); }
+
 typedef void (*gather_fn)(dlong, const dlong *, const dlong *, int, const void *, void *);
 typedef void (*gs_fn)(dlong, const dlong *, const dlong *, const dlong *, const dlong *, int, void *);
 typedef void (*scatter_fn)(dlong, const dlong *, const dlong *, int, const void *, void *);
 typedef void (*extract_fn)(dlong, int, const dlong *, const void *, void *);
 
-/* table order: [type][op] with op order Add, Mul, Max, Min (ogs.hpp:191) */
-static const gather_fn gather_tab[4][4] = {
+/* table order: [type][op] with op order Add, Mul, Max, Min (ogs.hpp:191);
+ * types 4 (bf16) and 5 (float, compensated Add) are the precision form,
+ * whose Mul entries are empty (not offered). */
+#define NTYPES 6
+static const gather_fn gather_tab[NTYPES][4] = {
     {(gather_fn)gather_f32_add, (gather_fn)gather_f32_mul, (gather_fn)gather_f32_max, (gather_fn)gather_f32_min},
     {(gather_fn)gather_f64_add, (gather_fn)gather_f64_mul, (gather_fn)gather_f64_max, (gather_fn)gather_f64_min},
     {(gather_fn)gather_i32_add, (gather_fn)gather_i32_mul, (gather_fn)gather_i32_max, (gather_fn)gather_i32_min},
-    {(gather_fn)gather_i64_add, (gather_fn)gather_i64_mul, (gather_fn)gather_i64_max, (gather_fn)gather_i64_min}};
-static const gs_fn gs_tab[4][4] = {
+    {(gather_fn)gather_i64_add, (gather_fn)gather_i64_mul, (gather_fn)gather_i64_max, (gather_fn)gather_i64_min},
+    {(gather_fn)gather_bf16_add, NULL, (gather_fn)gather_bf16_max, (gather_fn)gather_bf16_min},
+    {(gather_fn)gather_f32c_add, NULL, (gather_fn)gather_f32_max, (gather_fn)gather_f32_min}};
+static const gs_fn gs_tab[NTYPES][4] = {
     {(gs_fn)gs_f32_add, (gs_fn)gs_f32_mul, (gs_fn)gs_f32_max, (gs_fn)gs_f32_min},
     {(gs_fn)gs_f64_add, (gs_fn)gs_f64_mul, (gs_fn)gs_f64_max, (gs_fn)gs_f64_min},
     {(gs_fn)gs_i32_add, (gs_fn)gs_i32_mul, (gs_fn)gs_i32_max, (gs_fn)gs_i32_min},
-    {(gs_fn)gs_i64_add, (gs_fn)gs_i64_mul, (gs_fn)gs_i64_max, (gs_fn)gs_i64_min}};
-static const scatter_fn scatter_tab[4] = {(scatter_fn)scatter_f32, (scatter_fn)scatter_f64,
-                                          (scatter_fn)scatter_i32, (scatter_fn)scatter_i64};
-static const extract_fn extract_tab[4] = {(extract_fn)extract_f32, (extract_fn)extract_f64,
-                                          (extract_fn)extract_i32, (extract_fn)extract_i64};
+    {(gs_fn)gs_i64_add, (gs_fn)gs_i64_mul, (gs_fn)gs_i64_max, (gs_fn)gs_i64_min},
+    {(gs_fn)gs_bf16_add, NULL, (gs_fn)gs_bf16_max, (gs_fn)gs_bf16_min},
+    {(gs_fn)gs_f32c_add, NULL, (gs_fn)gs_f32_max, (gs_fn)gs_f32_min}};
+static const scatter_fn scatter_tab[NTYPES] = {(scatter_fn)scatter_f32, (scatter_fn)scatter_f64,
+                                               (scatter_fn)scatter_i32, (scatter_fn)scatter_i64,
+                                               (scatter_fn)scatter_u16, (scatter_fn)scatter_f32};
+static const extract_fn extract_tab[NTYPES] = {(extract_fn)extract_f32, (extract_fn)extract_f64,
+                                               (extract_fn)extract_i32, (extract_fn)extract_i64,
+                                               (extract_fn)extract_u16, (extract_fn)extract_f32};
 
 static int bad_args(int type, int opr, int trans, int K) {
-  return type < 0 || type > 3 || opr < 0 || opr > 3 || trans < 0 || trans > 2 || K < 1;
+  return type < 0 || type >= NTYPES || opr < 0 || opr > 3 || trans < 0 || trans > 2 || K < 1;
 }
 
 /* Gather list selection (71-79): NoTrans -> N rows/lists, else T */
 int oracle_op_gather(const oracle_op_t *op, int type, int opr, int trans, int K,
                      const void *v, void *gv) {
   if (bad_args(type, opr, trans, K)) return OR_E_ARG;
+  if (!gather_tab[type][opr]) return OR_E_NOT_SUPPORTED;
   if (trans == OR_NOTRANS)
     gather_tab[type][opr](op->NrowsN, op->rowStartsN, op->colIdsN, K, v, gv);
   else
@@ -484,6 +618,7 @@ int oracle_op_scatter(const oracle_op_t *op, int type, int trans, int K, const v
 /* GatherScatter list selection (321-339) */
 int oracle_op_gather_scatter(const oracle_op_t *op, int type, int opr, int trans, int K, void *v) {
   if (bad_args(type, opr, trans, K)) return OR_E_ARG;
+  if (!gs_tab[type][opr]) return OR_E_NOT_SUPPORTED;
   if (trans == OR_TRANS)
     gs_tab[type][opr](op->NrowsN, op->rowStartsT, op->colIdsT, op->rowStartsN, op->colIdsN, K, v);
   else if (trans == OR_SYM)

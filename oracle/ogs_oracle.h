@@ -20,7 +20,10 @@ typedef int32_t dlong;  /* include/types.h:60 */
 typedef int64_t hlong;  /* include/types.h:52 */
 
 /* enum orders follow include/ogs.hpp:179-200 */
-enum { OR_FLOAT = 0, OR_DOUBLE = 1, OR_INT32 = 2, OR_INT64 = 3 };
+enum { OR_FLOAT = 0, OR_DOUBLE = 1, OR_INT32 = 2, OR_INT64 = 3,
+       /* precision form (the library's own types, no reference counterpart):
+        * bfloat16 storage / float storage, Add with two-sum compensation */
+       OR_BF16 = 4, OR_FLOAT_C = 5 };
 enum { OR_ADD = 0, OR_MUL = 1, OR_MAX = 2, OR_MIN = 3 };
 enum { OR_SYM = 0, OR_NOTRANS = 1, OR_TRANS = 2 };
 enum { OR_UNSIGNED = 0, OR_SIGNED = 1, OR_HALO = 2 };
@@ -32,7 +35,8 @@ enum {
   OR_E_HALO_POSITIVE = -2,   /* ogsSetup.cpp:286-288 */
   OR_E_ROW_TOO_LONG = -3,    /* ogsOperator.cpp:460-465 */
   OR_E_GATHER_UNDEF = -4,    /* ogsSetup.cpp:861-864 */
-  OR_E_ARG = -5
+  OR_E_ARG = -5,
+  OR_E_NOT_SUPPORTED = -6    /* Mul on the precision-form types */
 };
 
 /* One CSR gather operator: ogsOperator_t's host members (ogsOperator.hpp:43-64). */
@@ -85,6 +89,10 @@ int oracle_scatter(const oracle_ogs_t *ogs, int type, int trans, int K,
 /* dawgsMain.cpp:326-330: std::mt19937(seed), uniform_real_distribution(-0.25,0.25) */
 void oracle_reference_q(uint32_t seed, dlong n, double *q);
 
+/* bfloat16 <-> float as the precision form defines them (round to nearest
+ * even, NaN quieted) */
+float oracle_bf16_to_float(uint16_t b);
+uint16_t oracle_float_to_bf16(float f);
 /* OpenMP thread count actually used by the host loops */
 int oracle_num_threads(void);
 

@@ -21,12 +21,26 @@ REF_PATH = os.path.join(HERE, "_ref", "libgs_ref.so")
 
 # enum orders: include/ogs.hpp:179-200
 FLOAT, DOUBLE, INT32, INT64 = 0, 1, 2, 3
+# precision form (the library's own types): bfloat16 storage (numpy uint16
+# bit patterns) and float storage with compensated Add (numpy float32)
+BF16, FLOAT_C = 4, 5
 ADD, MUL, MAX, MIN = 0, 1, 2, 3
 SYM, NOTRANS, TRANS = 0, 1, 2
 UNSIGNED, SIGNED, HALO = 0, 1, 2
 
-NP_TYPES = {FLOAT: np.float32, DOUBLE: np.float64, INT32: np.int32, INT64: np.int64}
+NP_TYPES = {FLOAT: np.float32, DOUBLE: np.float64, INT32: np.int32, INT64: np.int64, BF16: np.uint16}
 TYPE_OF = {np.dtype(v): k for k, v in NP_TYPES.items()}
+NP_TYPES[FLOAT_C] = np.float32
+
+
+def _ty(v: np.ndarray, type) -> int:
+    """Oracle type code: `type` when given (FLOAT_C for float32 storage with
+    compensated Add), else the one the array's dtype names."""
+    if type is None:
+        return TYPE_OF[v.dtype]
+    if np.dtype(NP_TYPES[type]) != v.dtype:
+        raise TypeError(f"type {type} stores {np.dtype(NP_TYPES[type])}, got {v.dtype}")
+    return type
 
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
@@ -75,6 +89,10 @@ def lib():
         L.oracle_scatter.argtypes = [C.POINTER(OracleOgsStruct), C.c_int, C.c_int, C.c_int, _P, _P]
         L.oracle_reference_q.argtypes = [C.c_uint32, C.c_int32, _P]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_bf16_to_float.argtypes = [C.c_uint16]
+        L.oracle_bf16_to_float.restype = C.c_float
+        L.oracle_float_to_bf16.argtypes = [C.c_float]
+        L.oracle_float_to_bf16.restype = C.c_uint16
         _lib = L
     return _lib
 
@@ -164,41 +182,41 @@ class OracleOgs:
         except Exception:
             pass
 
-    def gather_scatter(self, v: np.ndarray, K: int, op: int, trans: int) -> np.ndarray:
+    def gather_scatter(self, v: np.ndarray, K: int, op: int, trans: int, type=None) -> np.ndarray:
         v = np.ascontiguousarray(v)
-        _check(lib().oracle_gather_scatter(C.byref(self.s), TYPE_OF[v.dtype], op, trans, K, _ptr(v)))
+        _check(lib().oracle_gather_scatter(C.byref(self.s), _ty(v, type), op, trans, K, _ptr(v)))
         return v
 
-    def gather(self, v: np.ndarray, K: int, op: int, trans: int) -> np.ndarray:
+    def gather(self, v: np.ndarray, K: int, op: int, trans: int, type=None) -> np.ndarray:
         v = np.ascontiguousarray(v)
         nrows = self.NrowsN if trans == NOTRANS else self.NrowsT
         gv = np.zeros(max(self.Ngather, self.NlocalT) * K, dtype=v.dtype)
-        _check(lib().oracle_gather(C.byref(self.s), TYPE_OF[v.dtype], op, trans, K, _ptr(v), _ptr(gv)))
+        _check(lib().oracle_gather(C.byref(self.s), _ty(v, type), op, trans, K, _ptr(v), _ptr(gv)))
         return gv[: nrows * K] if trans == NOTRANS else gv[: self.NlocalT * K]
 
-    def scatter(self, v: np.ndarray, gv: np.ndarray, K: int, trans: int) -> np.ndarray:
+    def scatter(self, v: np.ndarray, gv: np.ndarray, K: int, trans: int, type=None) -> np.ndarray:
         v = np.ascontiguousarray(v)
         gv = np.ascontiguousarray(gv, dtype=v.dtype)
-        _check(lib().oracle_scatter(C.byref(self.s), TYPE_OF[v.dtype], trans, K, _ptr(gv), _ptr(v)))
+        _check(lib().oracle_scatter(C.byref(self.s), _ty(v, type), trans, K, _ptr(gv), _ptr(v)))
         return v
 
     # operator-level access (ogsOperator_t host templates), no gather_defined check
-    def op_gather(self, v, K, op, trans):
+    def op_gather(self, v, K, op, trans, type=None):
         v = np.ascontiguousarray(v)
         nrows = self.NrowsN if trans == NOTRANS else self.NrowsT
         gv = np.zeros(max(nrows, 1) * K, dtype=v.dtype)
-        _check(lib().oracle_op_gather(C.byref(self.s.local), TYPE_OF[v.dtype], op, trans, K, _ptr(v), _ptr(gv)))
+        _check(lib().oracle_op_gather(C.byref(self.s.local), _ty(v, type), op, trans, K, _ptr(v), _ptr(gv)))
         return gv[: nrows * K]
 
-    def op_scatter(self, v, gv, K, trans):
+    def op_scatter(self, v, gv, K, trans, type=None):
         v = np.ascontiguousarray(v)
         gv = np.ascontiguousarray(gv, dtype=v.dtype)
-        _check(lib().oracle_op_scatter(C.byref(self.s.local), TYPE_OF[v.dtype], trans, K, _ptr(gv), _ptr(v)))
+        _check(lib().oracle_op_scatter(C.byref(self.s.local), _ty(v, type), trans, K, _ptr(gv), _ptr(v)))
         return v
 
-    def op_gather_scatter(self, v, K, op, trans):
+    def op_gather_scatter(self, v, K, op, trans, type=None):
         v = np.ascontiguousarray(v)
-        _check(lib().oracle_op_gather_scatter(C.byref(self.s.local), TYPE_OF[v.dtype], op, trans, K, _ptr(v)))
+        _check(lib().oracle_op_gather_scatter(C.byref(self.s.local), _ty(v, type), op, trans, K, _ptr(v)))
         return v
 
 
@@ -222,22 +240,22 @@ class OracleCsr:
         op.colIdsN = self.ciN.ctypes.data_as(_I32P)
         self.op = op
 
-    def gather_scatter(self, v, K, op, trans):
+    def gather_scatter(self, v, K, op, trans, type=None):
         v = np.ascontiguousarray(v)
-        _check(lib().oracle_op_gather_scatter(C.byref(self.op), TYPE_OF[v.dtype], op, trans, K, _ptr(v)))
+        _check(lib().oracle_op_gather_scatter(C.byref(self.op), _ty(v, type), op, trans, K, _ptr(v)))
         return v
 
-    def gather(self, v, K, op, trans):
+    def gather(self, v, K, op, trans, type=None):
         v = np.ascontiguousarray(v)
         nrows = self.op.NrowsN if trans == NOTRANS else self.op.NrowsT
         gv = np.zeros(max(nrows, 1) * K, dtype=v.dtype)
-        _check(lib().oracle_op_gather(C.byref(self.op), TYPE_OF[v.dtype], op, trans, K, _ptr(v), _ptr(gv)))
+        _check(lib().oracle_op_gather(C.byref(self.op), _ty(v, type), op, trans, K, _ptr(v), _ptr(gv)))
         return gv[: nrows * K]
 
-    def scatter(self, v, gv, K, trans):
+    def scatter(self, v, gv, K, trans, type=None):
         v = np.ascontiguousarray(v)
         gv = np.ascontiguousarray(gv, dtype=v.dtype)
-        _check(lib().oracle_op_scatter(C.byref(self.op), TYPE_OF[v.dtype], trans, K, _ptr(gv), _ptr(v)))
+        _check(lib().oracle_op_scatter(C.byref(self.op), _ty(v, type), trans, K, _ptr(gv), _ptr(v)))
         return v
 
 
@@ -246,12 +264,27 @@ def csr_gather_scatter(rowStarts, colIds, v, K, op):
     return OracleCsr(rowStarts, colIds).gather_scatter(v, K, op, SYM)
 
 
-def extract(ids: np.ndarray, q: np.ndarray, K: int) -> np.ndarray:
+def extract(ids: np.ndarray, q: np.ndarray, K: int, type=None) -> np.ndarray:
     ids = np.ascontiguousarray(ids, dtype=np.int32)
     q = np.ascontiguousarray(q)
     gq = np.empty(len(ids) * K, dtype=q.dtype)
-    _check(lib().oracle_extract(TYPE_OF[q.dtype], len(ids), K, _ptr(ids), _ptr(q), _ptr(gq)))
+    _check(lib().oracle_extract(_ty(q, type), len(ids), K, _ptr(ids), _ptr(q), _ptr(gq)))
     return gq
+
+
+def bf16_to_float(b: np.ndarray) -> np.ndarray:
+    """bfloat16 bit patterns (uint16) -> float32, exact."""
+    return (np.asarray(b, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def float_to_bf16(f: np.ndarray) -> np.ndarray:
+    """float32 -> bfloat16 bit patterns, round to nearest even, NaN quieted
+    (numpy restatement of oracle_float_to_bf16)."""
+    u = np.ascontiguousarray(f, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return r
 
 
 class GsRef:
