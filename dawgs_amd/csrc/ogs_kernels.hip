@@ -174,6 +174,45 @@ template <> struct Red<bf16s, OGS_MIN> {
   static __device__ __forceinline__ bf16s fin(A a) { return bf16_round(a); }
 };
 
+// LDS staging of values in the entry-major kernels: bfloat16 is staged
+// widened to float, so that every LDS access is a whole dword (2-byte LDS
+// writes to the two halves of one dword collide); other types as stored.
+// LRed reduces staged values exactly as Red reduces stored ones.
+template <typename T> struct Lds {
+  using L = T;
+  static __device__ __forceinline__ L in(T x) { return x; }
+  static __device__ __forceinline__ T out(L x) { return x; }
+};
+template <> struct Lds<bf16s> {
+  using L = float;
+  static __device__ __forceinline__ float in(bf16s x) { return bf16_widen(x); }
+  static __device__ __forceinline__ bf16s out(float f) { return bf16s{(uint16_t)(__float_as_uint(f) >> 16)}; }
+};
+template <typename T, int OP> struct LRed {
+  using A = typename Red<T, OP>::A;
+  static __device__ __forceinline__ A init() { return Red<T, OP>::init(); }
+  static __device__ __forceinline__ void apply(A &a, T x) { Red<T, OP>::apply(a, x); }
+  static __device__ __forceinline__ T fin(const A &a) { return Red<T, OP>::fin(a); }
+};
+template <> struct LRed<bf16s, OGS_ADD> {
+  using A = Comp;
+  static __device__ __forceinline__ A init() { return Comp{0.0f, 0.0f}; }
+  static __device__ __forceinline__ void apply(A &a, float x) { comp_add(a, x); }
+  static __device__ __forceinline__ float fin(const A &a) { return bf16_widen(bf16_round(comp_fin(a))); }
+};
+template <> struct LRed<bf16s, OGS_MAX> {
+  using A = float;
+  static __device__ __forceinline__ A init() { return -kBf16Max; }
+  static __device__ __forceinline__ void apply(A &a, float x) { a = (x > a) ? x : a; }
+  static __device__ __forceinline__ float fin(A a) { return a; }  // a stored value: already a bf16
+};
+template <> struct LRed<bf16s, OGS_MIN> {
+  using A = float;
+  static __device__ __forceinline__ A init() { return kBf16Max; }
+  static __device__ __forceinline__ void apply(A &a, float x) { a = (x < a) ? x : a; }
+  static __device__ __forceinline__ float fin(A a) { return a; }
+};
+
 // Whether (T, OP) has kernels: every op for the reference's types; Add,
 // Max and Min for the precision-form storage types.
 template <typename T, int OP>
@@ -731,7 +770,8 @@ struct VecShape {
   static constexpr int C = KT / VW;
   static constexpr int WORDS = (int)((sizeof(T) + 3) / 4);
   static constexpr int SL = VW * WORDS <= 3 ? 8 : VW * WORDS <= 6 ? 4 : 2;
-  static constexpr int CAP0 = SL * kBlock / C < vec_cap<T>() / KT ? SL * kBlock / C : vec_cap<T>() / KT;
+  using L = typename Lds<T>::L;
+  static constexpr int CAP0 = SL * kBlock / C < vec_cap<L>() / KT ? SL * kBlock / C : vec_cap<L>() / KT;
   static constexpr int CAP = CAP0 < kCap ? CAP0 : kCap;
   static_assert(KT % VW == 0, "field count must be a multiple of the item width");
 };
@@ -804,11 +844,45 @@ __device__ __forceinline__ void vec_gather(const VecStage<SL> &s, T (&x)[SL][Vec
   }
 }
 
+// Row heads as a bit mask, one bit per CSR position of the block, with every
+// position from n on marked (so the row of the last head ends at n).  Built
+// from the staged head bytes: word wd covers positions [32 wd, 32 wd + 32).
+__device__ __forceinline__ void head_mask(const uint8_t *__restrict__ head, int n, uint32_t *__restrict__ hm) {
+  const uint32_t *h4 = reinterpret_cast<const uint32_t *>(head);
+  for (int wd = threadIdx.x; wd <= (n >> 5); wd += kBlock) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t hb = h4[wd * 8 + q];  // four head bytes, each 0 or 1
+      bits |= ((hb & 1u) | ((hb >> 7) & 2u) | ((hb >> 14) & 4u) | ((hb >> 21) & 8u)) << (4 * q);
+    }
+    const int lo = n - wd * 32;  // positions >= n in this word
+    if (lo < 32) bits = (lo <= 0 ? 0u : bits) | (~0u << (lo > 0 ? lo : 0));
+    hm[wd] = bits;
+  }
+}
+
+// Members in the row whose head is at position i (the next marked position).
+__device__ __forceinline__ int row_len(const uint32_t *__restrict__ hm, int i) {
+  int wd = i >> 5;
+  uint32_t rest = (hm[wd] >> (i & 31)) >> 1;  // positions i+1 .. 32 wd + 31
+  int len = 1;
+  if (!rest) {
+    len = 32 - (i & 31);
+    while ((rest = hm[++wd]) == 0u) len += 32;
+    return len + __builtin_ctz(rest);
+  }
+  return len + __builtin_ctz(rest);
+}
+
 template <typename T, int OP, int KT, int SL>
 __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta &m,
-                                            const int32_t *__restrict__ starts,
-                                            const T (&x)[SL][VecShape<T, KT>::VW], T *__restrict__ val,
-                                            uint8_t *__restrict__ head) {
+                                           const int32_t *__restrict__ starts,
+                                           const T (&x)[SL][VecShape<T, KT>::VW],
+                                           typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                           uint32_t *__restrict__ hm) {
+  using L = typename Lds<T>::L;
+  using R = LRed<T, OP>;
   constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
   const int n = s.n;
   if (n == 0) return;
@@ -823,7 +897,7 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
         const int c = it % C;
         const int ps = (int)(s.w[j] & kPackPosMask);
 #pragma unroll
-        for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = x[j][v];
+        for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = Lds<T>::in(x[j][v]);
         if (c == 0) head[ps] = (uint8_t)((s.w[j] >> kPackHeadBit) & 1u);
       }
     }
@@ -838,7 +912,7 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
       if (it < ni) {
         const int c = it % C, i = it / C;
 #pragma unroll
-        for (int v = 0; v < VW; ++v) val[i * KT + c * VW + v] = x[j][v];
+        for (int v = 0; v < VW; ++v) val[i * KT + c * VW + v] = Lds<T>::in(x[j][v]);
         if (c == 0) head[i] = 0;
       }
     }
@@ -851,18 +925,41 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
     }
   }
   __syncthreads();
-  for (int t = tid; t < n * KT; t += kBlock) {
-    const int i = t / KT, k = t - i * KT;
-    if (head[i]) {
-      typename Red<T, OP>::A red = Red<T, OP>::init();
-      int mm = i;
-      do {
-        Red<T, OP>::apply(red, val[mm * KT + k]);
-        ++mm;
-      } while (mm < n && !head[mm]);
-      const T acc = Red<T, OP>::fin(red);
-      for (int r = i; r < mm; ++r) val[r * KT + k] = acc;
+  head_mask(head, n, hm);
+  __syncthreads();
+  // one lane per (row head, field chunk): the row's VW fields reduced in CSR
+  // order, members read four at a time
+  for (int t = tid; t < ni; t += kBlock) {
+    const int i = t / C, c = t - i * C;
+    if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
+    const int len = row_len(hm, i);
+    typename R::A red[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[v] = R::init();
+    const L *src = val + i * KT + c * VW;
+    int mb = 0;
+    for (; mb + 4 <= len; mb += 4) {
+      L y[4][VW];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) y[u][v] = src[(mb + u) * KT + v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) R::apply(red[v], y[u][v]);
     }
+    for (; mb < len; ++mb) {
+#pragma unroll
+      for (int v = 0; v < VW; ++v) R::apply(red[v], src[mb * KT + v]);
+    }
+    L res[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) res[v] = R::fin(red[v]);
+    L *dst = val + i * KT + c * VW;
+    for (int r = 0; r < len; ++r)
+#pragma unroll
+      for (int v = 0; v < VW; ++v) dst[r * KT + v] = res[v];
   }
   __syncthreads();
   T *q = static_cast<T *>(s.src.q);
@@ -874,7 +971,7 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
       const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
       T *d = q + (size_t)slot_col(s.w[j], s.bb, packed) * KT + c * VW;
 #pragma unroll
-      for (int v = 0; v < VW; ++v) d[v] = val[ps * KT + c * VW + v];
+      for (int v = 0; v < VW; ++v) d[v] = Lds<T>::out(val[ps * KT + c * VW + v]);
     }
   }
 }
@@ -887,8 +984,11 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
                                                        const int4 *__restrict__ base, int gUseN,
                                                        T *__restrict__ q) {
   using V = VecShape<T, KT>;
-  __shared__ T val[vec_cap<T>()];
-  __shared__ uint8_t head[kCap];
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const VecSrc src = {part, ids, pk, base, q, gUseN};
   VecStage<V::SL> s;
   VecMeta m;
@@ -896,7 +996,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_meta(src, xcd_block(blockIdx.x, gridDim.x), m);
   vec_index<T, KT>(s, src, m);
   vec_gather<T, KT>(s, x);
-  vec_finish<T, OP, KT>(s, m, starts, x, val, head);
+  vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
 }
 
 // Batched form: one (descriptor, block) item of one field-count class per
@@ -905,8 +1005,11 @@ template <typename T, int OP, int KT>
 __global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict__ descs,
                                                       const int2 *__restrict__ items) {
   using V = VecShape<T, KT>;
-  __shared__ T val[vec_cap<T>()];
-  __shared__ uint8_t head[kCap];
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const int2 w = items[xcd_block(blockIdx.x, gridDim.x)];
   const BatchDev d = load_desc(descs, w.x);
   const VecSrc src = {d.part, d.G.ids, d.G.pk, d.G.base, d.q, 0};
@@ -916,7 +1019,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict
   vec_meta(src, w.y, m);
   vec_index<T, KT>(s, src, m);
   vec_gather<T, KT>(s, x);
-  vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head);
+  vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head, hm);
 }
 
 // ---------------------------------------------------------------------------
