@@ -27,7 +27,7 @@ class BatchDesc(C.Structure):
 
 
 class BatchInfo(C.Structure):
-    _fields_ = [("items", C.c_int32), ("nodes", C.c_int32), ("narrow", C.c_int32), ("order", C.c_int32),
+    _fields_ = [("items", C.c_int32), ("nodes", C.c_int32), ("narrow", C.c_int32), ("order", C.c_int32), ("fused", C.c_int32),
                 ("candidates", C.c_int32), ("tune_ms", C.c_float), ("best_us", C.c_float)]
 
 

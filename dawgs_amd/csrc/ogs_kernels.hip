@@ -999,18 +999,12 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
 }
 
-// Batched form: one (descriptor, block) item of one field-count class per
-// workgroup; the items of each XCD form one contiguous run of the list.
+// Batched form: one (descriptor, block) item per workgroup.
 template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict__ descs,
-                                                      const int2 *__restrict__ items) {
+__device__ __forceinline__ void batch_item(const BatchDev *__restrict__ descs, int2 w,
+                                           typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                           uint32_t *__restrict__ hm) {
   using V = VecShape<T, KT>;
-  using L = typename Lds<T>::L;
-  __shared__ L val[vec_cap<L>()];
-  __shared__ uint32_t head4[kCap / 4];
-  __shared__ uint32_t hm[kCap / 32 + 1];
-  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
-  const int2 w = items[xcd_block(blockIdx.x, gridDim.x)];
   const BatchDev d = load_desc(descs, w.x);
   const VecSrc src = {d.part, d.G.ids, d.G.pk, d.G.base, d.q, 0};
   VecStage<V::SL> s;
@@ -1020,6 +1014,42 @@ __global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict
   vec_index<T, KT>(s, src, m);
   vec_gather<T, KT>(s, x);
   vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head, hm);
+}
+
+// One node per field-count class: the items of each XCD form one contiguous
+// run of the class's list.
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict__ descs,
+                                                      const int2 *__restrict__ items) {
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  batch_item<T, OP, KT>(descs, items[xcd_block(blockIdx.x, gridDim.x)], val, reinterpret_cast<uint8_t *>(head4), hm);
+}
+
+// All classes in one node: workgroup b runs item b of the concatenated class
+// lists (class c holds items [end[c-1], end[c])) with that class's body.
+struct ClassEnds {
+  int end[6];
+};
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_batch_fused(const BatchDev *__restrict__ descs,
+                                                        const int2 *__restrict__ items, ClassEnds ce) {
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int2 w = items[b];
+  if (b < ce.end[0]) batch_item<T, OP, 1>(descs, w, val, head, hm);
+  else if (b < ce.end[1]) batch_item<T, OP, 2>(descs, w, val, head, hm);
+  else if (b < ce.end[2]) batch_item<T, OP, 3>(descs, w, val, head, hm);
+  else if (b < ce.end[3]) batch_item<T, OP, 4>(descs, w, val, head, hm);
+  else if (b < ce.end[4]) batch_item<T, OP, 8>(descs, w, val, head, hm);
+  else batch_item<T, OP, 16>(descs, w, val, head, hm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1851,6 +1881,7 @@ namespace ogs_hip {
 struct BatchConfig {
   int narrow;  // K > 1 descriptors use the next narrower block width
   int order;   // 0: items of larger descriptors first, 1: descriptor order
+  int fused;   // 1: all field-count classes in one kernel node
 };
 
 // Field-count classes with a compile-time shape: K = 1, 2, 3, 4, 8, 16.
@@ -1945,14 +1976,23 @@ static void launch_batch_class(const BatchDev *d, const int2 *items, int n, hipS
 }
 
 template <typename T, int OP>
-static void launch_batch_op(const BatchDevTables &t, int cus, hipStream_t st) {
+static void launch_batch_op(const BatchDevTables &t, int cus, int fused, hipStream_t st) {
   const int2 *L = t.lists;
-  if (t.n[0]) launch_batch_class<T, OP, 1>(t.d, L + t.off[0], t.n[0], st);
-  if (t.n[1]) launch_batch_class<T, OP, 2>(t.d, L + t.off[1], t.n[1], st);
-  if (t.n[2]) launch_batch_class<T, OP, 3>(t.d, L + t.off[2], t.n[2], st);
-  if (t.n[3]) launch_batch_class<T, OP, 4>(t.d, L + t.off[3], t.n[3], st);
-  if (t.n[4]) launch_batch_class<T, OP, 8>(t.d, L + t.off[4], t.n[4], st);
-  if (t.n[5]) launch_batch_class<T, OP, 16>(t.d, L + t.off[5], t.n[5], st);
+  int nshaped = 0;
+  for (int c = 0; c < kPipeClasses; ++c) nshaped += t.n[c];
+  if (fused && nshaped) {
+    // the class lists are contiguous in t.lists, in class order
+    ClassEnds ce;
+    int e = 0;
+    for (int c = 0; c < kPipeClasses; ++c) ce.end[c] = (e += t.n[c]);
+    hipLaunchKernelGGL((k_batch_fused<T, OP>), dim3(nshaped), dim3(kBlock), 0, st, t.d, L, ce);
+  }
+  if (!fused && t.n[0]) launch_batch_class<T, OP, 1>(t.d, L + t.off[0], t.n[0], st);
+  if (!fused && t.n[1]) launch_batch_class<T, OP, 2>(t.d, L + t.off[1], t.n[1], st);
+  if (!fused && t.n[2]) launch_batch_class<T, OP, 3>(t.d, L + t.off[2], t.n[2], st);
+  if (!fused && t.n[3]) launch_batch_class<T, OP, 4>(t.d, L + t.off[3], t.n[3], st);
+  if (!fused && t.n[4]) launch_batch_class<T, OP, 8>(t.d, L + t.off[4], t.n[4], st);
+  if (!fused && t.n[5]) launch_batch_class<T, OP, 16>(t.d, L + t.off[5], t.n[5], st);
   // other K: persistent grid-stride kernels over their items
   static const int occ = occupancy_of(k_batch_gs<T, OP>);
   const int g = occ * cus;
@@ -1964,15 +2004,15 @@ static void launch_batch_op(const BatchDevTables &t, int cus, hipStream_t st) {
 }
 
 template <typename T>
-static int launch_batch_typed(int op, const BatchDevTables &t, int cus, hipStream_t st) {
+static int launch_batch_typed(int op, const BatchDevTables &t, int cus, int fused, hipStream_t st) {
   switch (op) {
-    case OGS_ADD: launch_batch_op<T, OGS_ADD>(t, cus, st); break;
+    case OGS_ADD: launch_batch_op<T, OGS_ADD>(t, cus, fused, st); break;
     case OGS_MUL:
-      if constexpr (kOffered<T, OGS_MUL>) launch_batch_op<T, OGS_MUL>(t, cus, st);
+      if constexpr (kOffered<T, OGS_MUL>) launch_batch_op<T, OGS_MUL>(t, cus, fused, st);
       else return OGS_E_NOT_SUPPORTED;
       break;
-    case OGS_MAX: launch_batch_op<T, OGS_MAX>(t, cus, st); break;
-    case OGS_MIN: launch_batch_op<T, OGS_MIN>(t, cus, st); break;
+    case OGS_MAX: launch_batch_op<T, OGS_MAX>(t, cus, fused, st); break;
+    case OGS_MIN: launch_batch_op<T, OGS_MIN>(t, cus, fused, st); break;
     default: return OGS_E_ARG;
   }
   return (int)hipGetLastError();
@@ -1991,8 +2031,8 @@ static int by_storage(int type, F &&f) {
   }
 }
 
-static int launch_batch(int type, int op, const BatchDevTables &t, int cus, hipStream_t st) {
-  return by_storage(type, [&](auto tag) { return launch_batch_typed<decltype(tag)>(op, t, cus, st); });
+static int launch_batch(int type, int op, const BatchDevTables &t, int cus, int fused, hipStream_t st) {
+  return by_storage(type, [&](auto tag) { return launch_batch_typed<decltype(tag)>(op, t, cus, fused, st); });
 }
 
 static int make_tables(int type, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
@@ -2029,7 +2069,7 @@ static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &o
 struct ogs_hip_batch_s {
   int type = 0, op = 0;
   ogs_hip::BatchDevTables tables;
-  ogs_hip::BatchConfig cfg = {0, 1};
+  ogs_hip::BatchConfig cfg = {0, 1, 1};
   int cus = 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2061,9 +2101,9 @@ static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t>
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
-  if (!rc) rc = launch_batch(type, op, dt, cus, st);
+  if (!rc) rc = launch_batch(type, op, dt, cus, c.fused, st);
   if (!rc) rc = (int)hipEventRecord(e0, st);
-  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt, cus, st);
+  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt, cus, c.fused, st);
   if (!rc) rc = (int)hipEventRecord(e1, st);
   if (!rc) rc = (int)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -2104,7 +2144,7 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!rc) rc = (int)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus < 1) cus = 1;
   b->cus = cus;
-  BatchConfig best = {0, 1};
+  BatchConfig best = {0, 1, 1};
   const auto t0 = std::chrono::steady_clock::now();
   if (!rc && tune && n > 0) {
     // scratch copies: tuning launches must not touch the caller's data
@@ -2116,11 +2156,11 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
       if (!rc) rc = (int)hipMemcpyAsync(scratch[i], in[i].d_v, bytes, hipMemcpyDeviceToDevice, st);
     }
     float bestUs = 0.f;
-    {
+    for (int fused = 0; fused <= 1; ++fused) {
       for (int narrow = 0; narrow <= (anyKv ? 1 : 0); ++narrow) {
         for (int order = 0; order <= 1; ++order) {
           if (rc) break;
-          const BatchConfig c = {narrow, order};
+          const BatchConfig c = {narrow, order, fused};
           float us = 0.f;
           rc = time_config(type, op, in, scratch, c, cus, st, &us);
           ++b->candidates;
@@ -2145,7 +2185,7 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!rc) rc = (int)hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (!rc) rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (!rc) {
-    const int lr = launch_batch(type, op, b->tables, cus, cs);
+    const int lr = launch_batch(type, op, b->tables, cus, best.fused, cs);
     const int er = (int)hipStreamEndCapture(cs, &b->graph);
     rc = lr ? lr : er;
   }
@@ -2173,11 +2213,13 @@ int ogs_hip_batch_destroy(ogs_hip_batch_t batch) {
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info) {
   if (!batch || !info) return OGS_E_ARG;
   info->items = batch->tables.total();
-  int nodes = 0;
-  for (int c = 0; c < ogs_hip::kPipeClasses + 2; ++c) nodes += batch->tables.n[c] > 0;
-  info->nodes = nodes;
+  int nodes = 0, shaped = 0;
+  for (int c = 0; c < ogs_hip::kPipeClasses; ++c) shaped += batch->tables.n[c] > 0;
+  for (int c = ogs_hip::kPipeClasses; c < ogs_hip::kPipeClasses + 2; ++c) nodes += batch->tables.n[c] > 0;
+  info->nodes = nodes + (batch->cfg.fused ? (shaped > 0) : shaped);
   info->narrow = batch->cfg.narrow;
   info->order = batch->cfg.order;
+  info->fused = batch->cfg.fused;
   info->candidates = batch->candidates;
   info->tune_ms = batch->tune_ms;
   info->best_us = batch->best_us;

@@ -130,10 +130,11 @@ int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
  * descriptor runs GatherScatter Sym with the batch's type and op on its
  * buffer d_v (Ncols*K values); buffers must not overlap.  The graph holds
  * one kernel node per field-count class present (K = 1, 2, 3, 4, 8, 16, each
- * with its compiled shape, one workgroup per (descriptor, block) item) and a
- * persistent grid-stride node for any other K.  Buffers are bound at
- * creation.  With tune != 0 the creation times candidate configurations
- * (block width of K > 1 classes, item order) on scratch copies of the
+ * with its compiled shape, one workgroup per (descriptor, block) item), or
+ * one node running all those classes, and a persistent grid-stride node for
+ * any other K.  Buffers are bound at creation.  With tune != 0 the creation
+ * times candidate configurations (one node or one per class, block width of
+ * K > 1 classes, item order) on scratch copies of the
  * buffers and keeps the fastest; the buffers themselves are not touched.
  * Creation allocates and synchronises; ogs_hip_batch_run only launches the
  * graph.  Concurrent runs of one batch race on its buffers. */
@@ -150,9 +151,10 @@ int ogs_hip_batch_destroy(ogs_hip_batch_t batch);
 /* What the creation chose: work items (blocks over all descriptors),
  * kernel nodes in the graph, whether K>1 classes use the next narrower
  * block width, item order (0 = larger descriptors first, 1 = descriptor
- * order), and the tuning time / best run. */
+ * order), whether all field-count classes share one kernel node (fused),
+ * and the tuning time / best run. */
 typedef struct {
-  int32_t items, nodes, narrow, order, candidates;
+  int32_t items, nodes, narrow, order, fused, candidates;
   float tune_ms, best_us;
 } ogs_hip_batch_info_t;
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info);
