@@ -44,10 +44,14 @@ struct Partition {
 
 // A list's entries in the blocks of one partition, each block's entries
 // sorted by column and packed; base[b] holds the block's segment bases, with
-// base[b].x < 0 for a block left in CSR order (more than kPackSegs segments).
+// base[b].x < 0 for a block left in CSR order (more than kPackSegs segments:
+// its words are then its column ids in CSR order).  Block b's words start at
+// b * stride (stride >= the partition's largest block, zero-padded), so a
+// kernel can load them knowing only b.
 struct Pack {
   uint32_t *pk = nullptr;
   int4 *base = nullptr;
+  int32_t stride = 0;  // words per block
   int32_t packed = 0;  // blocks packed
   int32_t plain = 0;   // blocks left in CSR order
 };

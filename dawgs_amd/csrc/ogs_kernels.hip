@@ -268,6 +268,7 @@ struct ListView {
   const int32_t *ids;
   const uint32_t *pk;
   const int4 *base;
+  int stride;  // pack words per block
 };
 
 // Index words of a block's list slots j (list positions tid + j*kBlock < n):
@@ -284,7 +285,7 @@ __device__ __forceinline__ bool slot_words(const ListView &L, int b, bool full, 
   bb = make_int4(-1, 0, 0, 0);
   if (L.pk && full) bb = L.base[b];
   const bool packed = bb.x >= 0;
-  const uint32_t *src = packed ? L.pk + g0 : reinterpret_cast<const uint32_t *>(L.ids) + g0;
+  const uint32_t *src = packed ? L.pk + (size_t)b * L.stride : reinterpret_cast<const uint32_t *>(L.ids) + g0;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const int i = tid + j * kBlock;
@@ -364,7 +365,7 @@ __device__ __forceinline__ void gs_sym_kv_block(const int4 *__restrict__ part, i
   int4 bb = make_int4(-1, 0, 0, 0);
   if (G.pk && e.full) bb = G.base[b];
   const bool packed = bb.x >= 0;
-  const uint32_t *src = packed ? G.pk + e.g0 : reinterpret_cast<const uint32_t *>(G.ids) + e.g0;
+  const uint32_t *src = packed ? G.pk + (size_t)b * G.stride : reinterpret_cast<const uint32_t *>(G.ids) + e.g0;
   uint32_t w[kSlots];
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
@@ -786,6 +787,7 @@ struct VecSrc {
   const int4 *base;
   void *q;
   int gUseN;
+  int stride;  // pack words per block
 };
 
 // A block's metadata: its partition entries and pack base.
@@ -810,20 +812,33 @@ __device__ __forceinline__ void vec_meta(const VecSrc &src, int b, VecMeta &m) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Index words of the block described by m (its loads are issued here).
+// Index words of block b.  With a pack the words sit at b * stride
+// (zero-padded past the block's entries), so their loads go out together
+// with the metadata loads instead of after them; otherwise the block's
+// colIds are read once its partition entry m has arrived.
 template <typename T, int KT, int SL>
-__device__ __forceinline__ void vec_index(VecStage<SL> &s, const VecSrc &src, const VecMeta &m) {
+__device__ __forceinline__ void vec_index(VecStage<SL> &s, const VecSrc &src, int b, const VecMeta &m) {
   constexpr int C = VecShape<T, KT>::C;
   s.src = src;
+  if (src.pk) {
+    const uint32_t *w0 = src.pk + (size_t)b * src.stride;
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int e = ((int)threadIdx.x + j * kBlock) / C;
+      s.w[j] = e < src.stride ? __builtin_nontemporal_load(w0 + e) : 0u;
+    }
+  }
   s.g0 = uniform(src.gUseN ? m.p0.z : m.p0.y);
   s.n = uniform(src.gUseN ? m.p1.z : m.p1.y) - s.g0;
   s.bb = make_int4(uniform(m.bb.x), uniform(m.bb.y), uniform(m.bb.z), uniform(m.bb.w));
-  const uint32_t *w0 = s.bb.x >= 0 ? src.pk + s.g0 : reinterpret_cast<const uint32_t *>(src.ids) + s.g0;
-  const int ni = s.n * C;
+  if (!src.pk) {
+    const uint32_t *w0 = reinterpret_cast<const uint32_t *>(src.ids) + s.g0;
+    const int ni = s.n * C;
 #pragma unroll
-  for (int j = 0; j < SL; ++j) {
-    const int it = (int)threadIdx.x + j * kBlock;
-    s.w[j] = it < ni ? __builtin_nontemporal_load(w0 + it / C) : 0u;
+    for (int j = 0; j < SL; ++j) {
+      const int it = (int)threadIdx.x + j * kBlock;
+      s.w[j] = it < ni ? __builtin_nontemporal_load(w0 + it / C) : 0u;
+    }
   }
 }
 
@@ -928,7 +943,9 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
   head_mask(head, n, hm);
   __syncthreads();
   // one lane per (row head, field chunk): the row's VW fields reduced in CSR
-  // order, members read four at a time
+  // order, members read four at a time, the result written back over the row
+  // (measured faster than every item summing its own row: rows of several
+  // members would be read once per member)
   for (int t = tid; t < ni; t += kBlock) {
     const int i = t / C, c = t - i * C;
     if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
@@ -981,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
                                                        const int32_t *__restrict__ starts,
                                                        const int32_t *__restrict__ ids,
                                                        const uint32_t *__restrict__ pk,
-                                                       const int4 *__restrict__ base, int gUseN,
+                                                       const int4 *__restrict__ base, int stride, int gUseN,
                                                        T *__restrict__ q) {
   using V = VecShape<T, KT>;
   using L = typename Lds<T>::L;
@@ -989,12 +1006,13 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   __shared__ uint32_t head4[kCap / 4];
   __shared__ uint32_t hm[kCap / 32 + 1];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
-  const VecSrc src = {part, ids, pk, base, q, gUseN};
+  const VecSrc src = {part, ids, pk, base, q, gUseN, stride};
   VecStage<V::SL> s;
   VecMeta m;
   T x[V::SL][V::VW];
-  vec_meta(src, xcd_block(blockIdx.x, gridDim.x), m);
-  vec_index<T, KT>(s, src, m);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  vec_meta(src, b, m);
+  vec_index<T, KT>(s, src, b, m);
   vec_gather<T, KT>(s, x);
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
 }
@@ -1006,12 +1024,12 @@ __device__ __forceinline__ void batch_item(const BatchDev *__restrict__ descs, i
                                            uint32_t *__restrict__ hm) {
   using V = VecShape<T, KT>;
   const BatchDev d = load_desc(descs, w.x);
-  const VecSrc src = {d.part, d.G.ids, d.G.pk, d.G.base, d.q, 0};
+  const VecSrc src = {d.part, d.G.ids, d.G.pk, d.G.base, d.q, 0, d.G.stride};
   VecStage<V::SL> s;
   VecMeta m;
   T x[V::SL][V::VW];
   vec_meta(src, w.y, m);
-  vec_index<T, KT>(s, src, m);
+  vec_index<T, KT>(s, src, w.y, m);
   vec_gather<T, KT>(s, x);
   vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head, hm);
 }
@@ -1267,7 +1285,7 @@ __global__ __launch_bounds__(kBlock) void k_part_max(const int4 *__restrict__ pa
 // keeps base.x = -1 and is read in CSR order.
 __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ part, int nrows,
                                                        const int32_t *__restrict__ starts,
-                                                       const int32_t *__restrict__ ids, int useN,
+                                                       const int32_t *__restrict__ ids, int useN, int stride,
                                                        uint32_t *__restrict__ pk, int4 *__restrict__ base) {
   __shared__ int32_t key[kCap];
   __shared__ int16_t val[kCap];
@@ -1282,7 +1300,8 @@ __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ 
     g0 = useN ? p0.z : p0.y;
     n = ((r1 == p1.x) ? (useN ? p1.z : p1.y) : starts[r1]) - g0;
   }
-  if (n <= 0 || n > kCap) {
+  uint32_t *out = pk + (size_t)b * stride;
+  if (n <= 0 || n > kCap || n > stride) {
     if (tid == 0) base[b] = make_int4(-1, 0, 0, 0);
     return;
   }
@@ -1332,15 +1351,16 @@ __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ 
     fail = bad;
   }
   __syncthreads();
-  if (fail) {
+  if (fail) {  // CSR order: the block's column ids
+    for (int t = tid; t < n; t += kBlock) out[t] = (uint32_t)ids[g0 + t];
     if (tid == 0) base[b] = make_int4(-1, 0, 0, 0);
     return;
   }
   for (int t = tid; t < n; t += kBlock) {
     int sg = 0;
     for (int s = 1; s < nseg; ++s) sg += (t >= segStart[s]);
-    pk[g0 + t] = ((uint32_t)sg << kPackSegShift) | ((uint32_t)(key[t] - segBase[sg]) << kPackOffShift) |
-                 ((uint32_t)head[val[t]] << kPackHeadBit) | (uint32_t)val[t];
+    out[t] = ((uint32_t)sg << kPackSegShift) | ((uint32_t)(key[t] - segBase[sg]) << kPackOffShift) |
+             ((uint32_t)head[val[t]] << kPackHeadBit) | (uint32_t)val[t];
   }
   if (tid == 0) {
     int bs[kPackSegs] = {0, 0, 0, 0};
@@ -1403,9 +1423,11 @@ static ListView list_view(const Plan *p, bool listN, const Partition *pt, bool p
       else if (j == 0) pk = &p->packNT;
     }
   }
+  L.stride = 0;
   if (pk && pk->pk) {
     L.pk = pk->pk;
     L.base = pk->base;
+    L.stride = pk->stride;
   }
   return L;
 }
@@ -1442,7 +1464,7 @@ static const Partition *vec_partition(const Plan *p, int K, int narrow) {
 template <typename T, int OP, int KT>
 static void launch_vec_k(const Partition *pt, const ListView &G, int gN, T *v, hipStream_t st) {
   hipLaunchKernelGGL((k_gs_sym_vec<T, OP, KT>), dim3(pt->nblocks), dim3(kBlock), 0, st, pt->d, G.starts, G.ids, G.pk,
-                     G.base, gN, v);
+                     G.base, G.stride, gN, v);
 }
 
 template <typename T, int OP>
@@ -1659,11 +1681,16 @@ static int build_partition(Plan *p, bool cutOnN, int nrows, int W, int32_t *scra
 static int build_pack(const Partition &pt, int nrows, const int32_t *starts, const int32_t *ids, bool useN,
                       int nnz, int32_t *scratch, hipStream_t st, Pack *out) {
   if (pt.nblocks <= 0 || nnz <= 0) return OGS_SUCCESS;
-  hipError_t e = hipMalloc(&out->pk, sizeof(uint32_t) * (size_t)nnz);
+  // fixed stride: the largest block of this list, rounded up to a wave
+  const int mx = useN ? pt.maxN : pt.maxT;
+  out->stride = ((mx > 0 ? mx : 1) + 63) / 64 * 64;
+  const size_t words = (size_t)pt.nblocks * (size_t)out->stride;
+  hipError_t e = hipMalloc(&out->pk, sizeof(uint32_t) * words);
   if (e == hipSuccess) e = hipMalloc(&out->base, sizeof(int4) * (size_t)pt.nblocks);
+  if (e == hipSuccess) e = hipMemsetAsync(out->pk, 0, sizeof(uint32_t) * words, st);
   if (e != hipSuccess) return (int)e;
   hipLaunchKernelGGL(k_build_pack, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, nrows, starts, ids, useN ? 1 : 0,
-                     out->pk, out->base);
+                     out->stride, out->pk, out->base);
   e = hipMemsetAsync(scratch, 0, sizeof(int32_t), st);
   if (e != hipSuccess) return (int)e;
   hipLaunchKernelGGL(k_count_packed, dim3(grid_for(pt.nblocks, 1024)), dim3(kBlock), 0, st, out->base, pt.nblocks,
