@@ -11,7 +11,12 @@ then K timed steps between synchronisations, timed with HIP events on the
 stream the kernel is launched on.  DOFS/s follows the reference's
 definition: NgatherGlobal / time per call (dawgsMain.cpp:207).
 
-Prints ONE JSON line on stdout; diagnostics go to stderr.
+Prints ONE JSON line on stdout; diagnostics go to stderr.  After the timed
+region the default line also carries summaries of the metric's other half,
+"batched/fp32 vs plain": `precision` (FP32-compensated and bf16 storage, K=3
+on the same mesh, against the plain fp64 method, with errors against the fp64
+oracle) and `batched` (48 meshes in one graph launch against one launch per
+mesh and field); --no-extras skips them.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 100] [--nx 48] [--p 7] [--no-cpu] [--copy-probe]
     python bench.py --sweep [DEGREES] | --batched | --precision
@@ -159,7 +164,7 @@ def batch_meshes(seed: int = 48, n: int = 48):
     return out
 
 
-def run_batched(args) -> int:
+def batched_result(args) -> dict:
     import torch
     import dawgs_amd as D
     from dawgs_amd.mesh import make_ids
@@ -210,7 +215,11 @@ def run_batched(args) -> int:
            "roofline": {"bound": "hbm", "algorithmic_bytes": B, "achieved": round(B / t_batch / 1e9, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(B / t_batch / 1e9 / HBM_PEAK_GBS, 4)},
            "autotune": info}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def run_batched(args) -> int:
+    print(json.dumps(batched_result(args)), flush=True)
     return 0
 
 
@@ -225,7 +234,7 @@ def error_stats(res: np.ndarray, y: np.ndarray) -> dict:
             "max_elem_rel": float(np.max(np.abs(d[big]) / np.abs(y[big])))}
 
 
-def run_precision(args) -> int:
+def precision_result(args) -> dict:
     """Precision form: FLOAT_C (float storage) and BF16 storage, Add reduced
     in float with two-sum compensation, K=3 interleaved fields on the 48^3
     p=7 mesh, values log-uniform in magnitude over [1e-3, 1e3] with random
@@ -333,7 +342,11 @@ def run_precision(args) -> int:
            "forms": res, "errors_vs_fp64": errors,
            "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": head["frac"], "algorithmic_bytes": head["algorithmic_bytes"]}}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def run_precision(args) -> int:
+    print(json.dumps(precision_result(args)), flush=True)
     return 0
 
 
@@ -346,6 +359,8 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--p", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: skip the batched and precision summaries of the default line")
     ap.add_argument("--sweep", nargs="?", const="1,2,3,4,5,6,7", default=None, metavar="DEGREES",
                     help="degree sweep at ~56 M local DOFs (48^3 at p=7): Device (and, unless --no-cpu, "
                          "Host) GatherScatter/Gather/Scatter lines in the reference's format; one JSON "
@@ -478,6 +493,32 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.no_extras:
+        # the metric's second half ("batched/fp32 vs plain"): the batched and
+        # precision forms, summarised; their full lines come from --batched /
+        # --precision.  A failure here leaves the headline line intact.
+        del q
+        ex = argparse.Namespace(**vars(args))
+        ex.warmup, ex.steps = 50, 50
+        try:
+            pr = precision_result(ex)
+            f = pr["forms"]
+            out["precision"] = {
+                "workload": pr["config"]["workload"],
+                "fp32_compensated_batched_vs_fp64_plain": pr["speedup_vs_fp64_plain"],
+                "fp32_compensated_batched_DOFSps": pr["value"],
+                "fp64_plain_per_field_DOFSps": f["fp64_plain_per_field"]["DOFSps"],
+                "bf16_vs_fp64_plain": f["bf16_k3"]["speedup_vs_fp64_plain"],
+                "errors_vs_fp64": pr["errors_vs_fp64"]}
+        except Exception as e:  # noqa: BLE001
+            out["precision"] = {"error": repr(e)}
+        try:
+            bt = batched_result(ex)
+            out["batched"] = {"workload": bt["config"]["workload"], "value": bt["value"], "unit": bt["unit"],
+                              "speedup_vs_plain": bt["speedup_vs_plain"], "ms_per_step": bt["ms_per_step"],
+                              "plain_ms_per_step": bt["plain"]["ms_per_step"], "roofline_frac": bt["roofline"]["frac"]}
+        except Exception as e:  # noqa: BLE001
+            out["batched"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
     return 0
 
