@@ -40,6 +40,12 @@ class OgsInfo(C.Structure):
                 ("NrowBlocksN", C.c_int32), ("NrowBlocksT", C.c_int32)]
 
 
+class HaloInfo(C.Structure):
+    _fields_ = [("N", C.c_int32), ("Ngather", C.c_int32), ("NlocalT", C.c_int32), ("NhaloT", C.c_int32),
+                ("NlocalP", C.c_int32), ("NhaloP", C.c_int32), ("Nhalo", C.c_int32),
+                ("NgatherGlobal", C.c_int64), ("gathered_halo", C.c_int32), ("gather_defined", C.c_int32)]
+
+
 # name -> (restype, argtypes); mirrors include/ogs_hip.h exactly
 SIGNATURES = {
     "ogs_hip_error_string": (C.c_char_p, [C.c_int]),
@@ -62,6 +68,12 @@ SIGNATURES = {
     "ogs_gather_scatter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P]),
     "ogs_gather": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P, _P]),
     "ogs_scatter": (C.c_int, [_P, C.c_int, C.c_int, _I32, _P, _P, _P]),
+    "halo_setup": (C.c_int, [C.POINTER(_P), _I32, _P, C.c_int, C.c_int]),
+    "halo_setup_from_gather": (C.c_int, [C.POINTER(_P), _P]),
+    "halo_free": (C.c_int, [_P]),
+    "halo_get_info": (C.c_int, [_P, C.POINTER(HaloInfo)]),
+    "halo_exchange": (C.c_int, [_P, C.c_int, _I32, _P, _P]),
+    "halo_combine": (C.c_int, [_P, C.c_int, _I32, _P, _P]),
 }
 
 _lib = None

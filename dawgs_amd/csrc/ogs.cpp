@@ -149,9 +149,6 @@ OGS_INSTANTIATE(float_c)
 
 using namespace libp;
 
-struct ogs_handle_s {
-  ogs::ogs_t ogs;
-};
 
 namespace {
 

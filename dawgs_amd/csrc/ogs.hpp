@@ -225,12 +225,62 @@ class ogs_t {
   template <typename T> void Scatter(memory<T> v, const memory<T> gv, const int k, const Transpose trans);
 
   const ogsOperator_t &local() const { return *gatherLocal; }
+  void AssertGatherDefined() const;  // ogsSetup.cpp:861-864
 
  private:
   std::shared_ptr<ogsOperator_t> gatherLocal;
   std::shared_ptr<ogsOperator_t> gatherHalo;
-  void AssertGatherDefined() const;  // ogsSetup.cpp:861-864
+  friend class halo_t;
+};
+
+// halo_t (ogs.hpp:347-397) at one rank.  A Halo-kind setup validates the ids
+// (every group needs exactly one positive id, ogsSetup.cpp:286-288) and
+// counts them; with a single rank no node is shared with another rank, so
+// the halo operator has no rows and the exchange no neighbours, and
+// Exchange/Combine move nothing (ogsHalo.cpp:39-359 with zero halo rows and
+// Nhalo = 0).  SetupFromGather (ogsSetup.cpp:888-914) takes the counts of a
+// gathered ogs_t, whose halo nodes would follow its NlocalT local rows.
+class halo_t {
+ public:
+  platform_t platform;
+  comm_t comm;
+
+  dlong N = 0;
+  dlong Ngather = 0;
+  dlong NlocalT = 0, NhaloT = 0, NlocalP = 0, NhaloP = 0;
+  hlong NgatherGlobal = 0;
+  Kind kind = Halo;
+  bool unique = false;
+  bool gather_defined = false;
+  bool gathered_halo = false;
+  dlong Nhalo = 0;  // extra received nodes, NhaloT - NhaloP
+
+  void Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Method method, const bool verbose,
+             platform_t &_platform);
+  void Setup(const dlong _N, hlong *ids, const Method method, const bool verbose, platform_t &_platform);
+  void SetupFromGather(ogs_t &ogs);
+  void Free();
+
+  template <typename T> void Exchange(deviceMemory<T> o_v, const int k);
+  template <typename T> void ExchangeStart(deviceMemory<T> o_v, const int k);
+  template <typename T> void ExchangeFinish(deviceMemory<T> o_v, const int k);
+  template <typename T> void Exchange(memory<T> v, const int k);
+
+  template <typename T> void Combine(deviceMemory<T> o_v, const int k);
+  template <typename T> void CombineStart(deviceMemory<T> o_v, const int k);
+  template <typename T> void CombineFinish(deviceMemory<T> o_v, const int k);
+  template <typename T> void Combine(memory<T> v, const int k);
+
+ private:
+  bool set_ = false;
+  std::shared_ptr<ogsOperator_t> gatherHalo;  // empty at one rank
+  void Check(const int k) const;
 };
 
 }  // namespace ogs
 }  // namespace libp
+
+// The opaque object behind ogs_handle_t (ogs.cpp, ogs_halo.cpp).
+struct ogs_handle_s {
+  libp::ogs::ogs_t ogs;
+};

@@ -219,3 +219,61 @@ class ogs_t:
     def Scatter(self, v, gv, k: int, trans: int, stream=None, type=None) -> None:
         check(lib().ogs_scatter(self.h, type_of(v, type), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
               "ogs_scatter")
+
+
+class halo_t:
+    """ogs::halo_t (include/ogs.hpp:347-397) at one rank: a Halo-kind setup
+    (exactly one positive id per group) whose Exchange / Combine move nothing,
+    since no node is shared with another rank."""
+
+    def __init__(self):
+        self.h = None
+        self._ogs = None
+
+    def _info(self):
+        i = _lib.HaloInfo()
+        check(lib().halo_get_info(self.h, C.byref(i)), "halo_get_info")
+        for n, _ in i._fields_:
+            setattr(self, n, getattr(i, n))
+        self.gathered_halo = bool(self.gathered_halo)
+        self.gather_defined = bool(self.gather_defined)
+
+    def Setup(self, N: int, ids: np.ndarray, method: int = Auto, verbose: bool = False) -> None:
+        """halo_t::Setup (ogsSetup.cpp:55-64)."""
+        if not (isinstance(ids, np.ndarray) and ids.dtype == np.int64 and ids.flags.c_contiguous):
+            raise TypeError("ids must be a contiguous numpy int64 array")
+        if N > len(ids):
+            raise ValueError("N exceeds len(ids)")
+        self.Free()
+        h = C.c_void_p()
+        check(lib().halo_setup(C.byref(h), N, ids.ctypes.data_as(C.c_void_p), method, int(verbose)), "halo_setup")
+        self.h = h
+        self._info()
+
+    def SetupFromGather(self, ogs: "ogs_t") -> None:
+        """halo_t::SetupFromGather (ogsSetup.cpp:888-914); `ogs` must be
+        gather-defined and outlive this halo."""
+        self.Free()
+        h = C.c_void_p()
+        check(lib().halo_setup_from_gather(C.byref(h), ogs.h), "halo_setup_from_gather")
+        self.h = h
+        self._ogs = ogs
+        self._info()
+
+    def Exchange(self, v, k: int, stream=None, type=None) -> None:
+        check(lib().halo_exchange(self.h, type_of(v, type), k, _dev_ptr(v), _stream(stream)), "halo_exchange")
+
+    def Combine(self, v, k: int, stream=None, type=None) -> None:
+        check(lib().halo_combine(self.h, type_of(v, type), k, _dev_ptr(v), _stream(stream)), "halo_combine")
+
+    def Free(self) -> None:
+        if self.h:
+            lib().halo_free(self.h)
+            self.h = None
+        self._ogs = None
+
+    def __del__(self):
+        try:
+            self.Free()
+        except Exception:
+            pass

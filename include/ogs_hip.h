@@ -203,6 +203,29 @@ int ogs_gather(ogs_handle_t h, int type, int op, int trans, int32_t K,
 int ogs_scatter(ogs_handle_t h, int type, int trans, int32_t K,
                 void *d_v, const void *d_gv, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* 3. halo_t at one rank (include/ogs.hpp:347-397)                    */
+/* halo_setup: halo_t::Setup (ogsSetup.cpp:55-64), a Halo-kind setup: every
+ * group of equal |id| needs exactly one positive id (else
+ * OGS_E_HALO_POSITIVE, ogsSetup.cpp:286-288).  halo_setup_from_gather:
+ * halo_t::SetupFromGather (ogsSetup.cpp:888-914) on a gather-defined
+ * (unique) ogs handle (else OGS_E_GATHER_UNDEF), which must outlive the
+ * halo.  Exchange / Combine (ogsHalo.cpp:39-359) on d_v (N*K values):
+ * at one rank no node is shared with another rank, so there are no halo
+ * rows and nothing moves (Nhalo = 0); they validate and return. */
+typedef struct halo_handle_s *halo_handle_t;
+typedef struct {
+  int32_t N, Ngather, NlocalT, NhaloT, NlocalP, NhaloP, Nhalo;
+  int64_t NgatherGlobal;
+  int32_t gathered_halo, gather_defined;
+} halo_info_t;
+int halo_setup(halo_handle_t *h, int32_t N, int64_t *ids, int method, int verbose);
+int halo_setup_from_gather(halo_handle_t *h, ogs_handle_t ogs);
+int halo_free(halo_handle_t h);
+int halo_get_info(halo_handle_t h, halo_info_t *info);
+int halo_exchange(halo_handle_t h, int type, int32_t K, void *d_v, void *stream);
+int halo_combine(halo_handle_t h, int type, int32_t K, void *d_v, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "ogs_hip.h")
 def declared_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(ogs_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b((?:ogs|halo)_\w+)\s*\(", src)))
 
 
 def test_header_declares_expected_entry_points():
@@ -54,6 +54,11 @@ def test_argument_checks_need_no_device():
     assert L.ogs_hip_extract(1, 0, 0, None, None, None, None) == -1
     assert L.ogs_gather_scatter(None, 1, 0, 0, 1, None, None) == -1
     assert L.ogs_setup(None, 0, None, 0, 0, 0, 0) == -1
+    assert L.halo_setup(None, 0, None, 0, 0) == -1
+    assert L.halo_setup_from_gather(None, None) == -1
+    assert L.halo_exchange(None, 1, 1, None, None) == -1
+    assert L.halo_combine(None, 1, 1, None, None) == -1
+    assert L.halo_free(None) == -1
 
 
 def test_python_mirror_enums_follow_reference():
