@@ -240,8 +240,8 @@ int ogs_get_csr(ogs_handle_t h, int32_t *rowStartsN, int32_t *colIdsN, int32_t *
   };
   cp(rowStartsN, L.rowStartsN, (size_t)L.NrowsT + 1);
   cp(rowStartsT, L.rowStartsT, (size_t)L.NrowsT + 1);
-  cp(colIdsN, L.colIdsN, (size_t)L.nnzN);
-  cp(colIdsT, L.colIdsT, (size_t)L.nnzT);
+  if (colIdsN && L.nnzN) cp(colIdsN, L.hostColIdsN(), (size_t)L.nnzN);
+  if (colIdsT && L.nnzT) cp(colIdsT, L.hostColIdsT(), (size_t)L.nnzT);
   cp(blockRowStartsN, L.blockRowStartsN, (size_t)L.NrowBlocksN + 1);
   cp(blockRowStartsT, L.blockRowStartsT, (size_t)L.NrowBlocksT + 1);
   return OGS_SUCCESS;

@@ -87,6 +87,14 @@ class deviceMemory {
     d.n_ = n;
     return d;
   }
+  // Takes ownership of a device allocation made elsewhere.
+  static deviceMemory adopt(T *p, size_t n) {
+    deviceMemory d;
+    d.own_ = std::shared_ptr<T>(p, [](T *q) { if (q) (void)hipFree(q); });
+    d.p_ = p;
+    d.n_ = n;
+    return d;
+  }
   T *ptr() const { return p_; }
   size_t length() const { return n_; }
   deviceMemory operator+(size_t off) const {
@@ -159,7 +167,8 @@ template <> struct ogsType<float_c> { static constexpr Type get() { return Float
 class ogsOperator_t {
  public:
   dlong Ncols = 0, NrowsN = 0, NrowsT = 0, nnzN = 0, nnzT = 0;
-  memory<dlong> rowStartsN, rowStartsT, colIdsN, colIdsT;
+  memory<dlong> rowStartsN, rowStartsT;
+  mutable memory<dlong> colIdsN, colIdsT;
   deviceMemory<dlong> o_rowStartsN, o_rowStartsT, o_colIdsN, o_colIdsT;
   dlong NrowBlocksN = 0, NrowBlocksT = 0;
   memory<dlong> blockRowStartsN, blockRowStartsT;
@@ -175,6 +184,11 @@ class ogsOperator_t {
 
   void setupRowBlocks();                 // ogsOperator.cpp:447-512
   void upload(const platform_t &platform);  // device copies + plan
+  void createPlan(const platform_t &platform);  // plan over the device lists
+  // Host colIds, copied from the device on first use when the setup built
+  // the lists there (only the CSR query reads them).
+  const memory<dlong> &hostColIdsT() const;
+  const memory<dlong> &hostColIdsN() const;
   ogs_hip_plan_t plan() const { return plan_; }
 
   template <typename T>

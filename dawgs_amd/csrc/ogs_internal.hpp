@@ -78,4 +78,20 @@ struct ogs_hip_plan_s {
 
 namespace ogs_hip {
 using Plan = ogs_hip_plan_s;
+
+// The single-rank setup on the device (ogs_setup_dev.hip): from host ids to
+// the local operator's CSR pair, device arrays handed to the caller (rsN and
+// ciN null for the Unsigned kind, whose N lists are the T lists; all null
+// for the Halo kind, whose local operator has no rows at one rank).
+struct DeviceSetup {
+  int32_t G = 0;        // groups (rows)
+  int32_t NlocalP = 0;  // groups with a positive member
+  int32_t nnzT = 0, nnzN = 0, maxRowT = 0, maxRowN = 0;
+  bool gather_defined = false;
+  int32_t *rsT = nullptr, *ciT = nullptr, *rsN = nullptr, *ciN = nullptr;
+};
+// kind: OGS_UNSIGNED / OGS_SIGNED / OGS_HALO; unique writes the signs back
+// into h_ids.  Returns OGS_E_HALO_POSITIVE for a Halo group without exactly
+// one positive id, or a hipError_t.
+int device_setup(int32_t N, int64_t *h_ids, int kind, int unique, hipStream_t st, DeviceSetup *out);
 }

@@ -1337,18 +1337,41 @@ __global__ __launch_bounds__(kBlock) void k_build_pack(const int4 *__restrict__ 
       __syncthreads();
     }
   }
+  // Segments: greedy runs of sorted columns spanning at most kPackOffMask.
+  // Segment s+1 starts at the first column past segBase[s] + kPackOffMask
+  // (a block-wide min over the sorted keys); at most kPackSegs of them.
+  __shared__ int32_t nextStart;
   if (tid == 0) {
-    int ns = 0, bad = 0;
-    for (int t = 0; t < n; ++t) {
-      if (ns == 0 || (uint32_t)(key[t] - segBase[ns - 1]) > kPackOffMask) {
-        if (ns == kPackSegs) { bad = 1; break; }
-        segStart[ns] = t;
-        segBase[ns] = key[t];
-        ++ns;
+    segStart[0] = 0;
+    segBase[0] = key[0];
+    nseg = 1;
+    fail = 0;
+  }
+  __syncthreads();
+  for (int sgi = 0; sgi < kPackSegs; ++sgi) {
+    if (tid == 0) nextStart = n;
+    __syncthreads();
+    const int a = segStart[sgi];
+    const uint32_t lim = (uint32_t)segBase[sgi];
+    for (int t = a + tid; t < n; t += kBlock)
+      if ((uint32_t)key[t] - lim > kPackOffMask) {
+        atomicMin(&nextStart, t);
+        break;  // keys ascend: this lane's later t are larger
+      }
+    __syncthreads();
+    const int ns = nextStart;
+    if (ns >= n) break;
+    if (tid == 0) {
+      if (sgi + 1 == kPackSegs) {
+        fail = 1;
+      } else {
+        segStart[sgi + 1] = ns;
+        segBase[sgi + 1] = key[ns];
+        nseg = sgi + 2;
       }
     }
-    nseg = ns;
-    fail = bad;
+    __syncthreads();
+    if (fail) break;
   }
   __syncthreads();
   if (fail) {  // CSR order: the block's column ids

@@ -6,64 +6,20 @@
 // order, positive groups first (ogsSetup.cpp:409-433); colIds ascending
 // within a row (637-663); N lists holding only the positive members;
 // Unsigned operators aliasing N to T (719, 728).  The method differs: with
-// one rank there is nothing to route, so instead of three sorts of 32-byte
-// nodes the setup makes two linear passes.  Groups are found by a direct
-// table over |id| when the ids are dense (the structured-mesh numbering) and
-// by an open-addressing hash otherwise, and are numbered in first-encounter
-// order, which is already the row order within each sign class.
+// one rank there is nothing to route, and the grouping, row numbering and
+// CSR construction run on the device (ogs_setup_dev.hip: one radix sort by
+// |id|, scans and scatters); the host receives the counts and its copies
+// of the lists.
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
 
 #include "ogs.hpp"
+#include "ogs_internal.hpp"
 
 namespace libp {
 namespace ogs {
-
-namespace {
-
-inline uint64_t uabs(hlong x) { return x < 0 ? (uint64_t)0 - (uint64_t)x : (uint64_t)x; }
-
-// |id| -> group number, first encounter order.  Returns the group count.
-dlong number_groups(const dlong N, const hlong *ids, std::vector<dlong> &grp) {
-  grp.assign((size_t)N, -1);
-  uint64_t maxAbs = 0;
-  for (dlong n = 0; n < N; ++n) {
-    const uint64_t a = uabs(ids[n]);
-    if (a > maxAbs) maxAbs = a;
-  }
-  dlong G = 0;
-  if (maxAbs <= 4 * (uint64_t)N + 1024) {
-    std::vector<dlong> table((size_t)maxAbs + 1, -1);
-    for (dlong n = 0; n < N; ++n) {
-      if (ids[n] == 0) continue;
-      dlong &slot = table[uabs(ids[n])];
-      if (slot < 0) slot = G++;
-      grp[n] = slot;
-    }
-    return G;
-  }
-  size_t cap = 16;
-  while (cap < 2 * (size_t)N) cap <<= 1;
-  std::vector<uint64_t> keys(cap, 0);  // 0 = empty (id 0 never enters)
-  std::vector<dlong> vals(cap, -1);
-  const size_t mask = cap - 1;
-  for (dlong n = 0; n < N; ++n) {
-    if (ids[n] == 0) continue;
-    const uint64_t key = uabs(ids[n]);
-    size_t h = (size_t)((key * 0x9E3779B97F4A7C15ull) >> 17) & mask;
-    while (keys[h] != 0 && keys[h] != key) h = (h + 1) & mask;
-    if (keys[h] == 0) {
-      keys[h] = key;
-      vals[h] = G++;
-    }
-    grp[n] = vals[h];
-  }
-  return G;
-}
-
-}  // namespace
 
 void ogs_t::Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kind _kind, const Method method,
                   const bool _unique, const bool verbose, platform_t &_platform) {
@@ -87,59 +43,23 @@ void ogs_t::Setup(const dlong _N, hlong *ids, const Kind _kind, const Method met
   LIBP_ABORT("Negative N", N < 0, OGS_E_ARG);
   LIBP_ABORT("Null ids", N > 0 && ids == nullptr, OGS_E_ARG);
 
-  std::vector<dlong> grp;
-  const dlong G = number_groups(N, ids, grp);
-
-  // Group sizes and positive-member counts.  Unsigned treats every id as
-  // positive (ogsSetup.cpp:115-116).
-  std::vector<dlong> gsize((size_t)G, 0), gpos((size_t)G, 0);
-  for (dlong n = 0; n < N; ++n) {
-    const dlong g = grp[n];
-    if (g < 0) continue;
-    gsize[g]++;
-    if (kind == Unsigned || ids[n] > 0) gpos[g]++;
-  }
-
-  if (unique) {
-    // One positive node per group: the first-encountered member.  The
-    // reference draws it with rand() (ogsSetup.cpp:264-275); ogs.hpp:96-99
-    // only asks for an arbitrary but consistent choice.  Signs are written
-    // back into ids (ogsSetup.cpp:139-140).
-    std::vector<char> seen((size_t)G, 0);
-    for (dlong n = 0; n < N; ++n) {
-      const dlong g = grp[n];
-      if (g < 0) continue;
-      const hlong a = (hlong)uabs(ids[n]);
-      ids[n] = seen[g] ? -a : a;
-      seen[g] = 1;
-      gpos[g] = 1;
-    }
-    gather_defined = true;
-  } else {
-    bool one = true;
-    for (dlong g = 0; g < G; ++g) {
-      if (gpos[g] != 1) one = false;
-      LIBP_ABORT("Found " << gpos[g] << " positive Ids for a Halo group.", kind == Halo && gpos[g] != 1,
-                 OGS_E_HALO_POSITIVE);                   // ogsSetup.cpp:286-288
-    }
-    gather_defined = one;                                // ogsSetup.cpp:312-313
-  }
+  // groups, signs, row order and the CSR pair, on the device
+  ogs_hip::DeviceSetup d;
+  const int rc = ogs_hip::device_setup(N, ids, (int)kind, unique ? 1 : 0, platform.stream, &d);
+  LIBP_ABORT("Found a Halo group without exactly one positive Id.", rc == OGS_E_HALO_POSITIVE,
+             OGS_E_HALO_POSITIVE);                       // ogsSetup.cpp:286-288
+  LIBP_ABORT("ogs device setup: " << ogs_hip_error_string(rc), rc != 0, rc);
+  gather_defined = d.gather_defined;                     // ogsSetup.cpp:312-313
+  const dlong G = d.G;
 
   // Rows: positive groups [0, NlocalP) then the others, each class in
   // first-encounter order (ogsSetup.cpp:409-433).  At one rank every group
   // is local: NhaloT = NhaloP = 0.
-  dlong nPos = 0;
-  for (dlong g = 0; g < G; ++g) nPos += (gpos[g] > 0);
-  NlocalP = nPos;
+  NlocalP = d.NlocalP;
   NlocalT = G;
   NhaloP = NhaloT = 0;
   Ngather = NlocalP + NhaloP;  // ogsSetup.cpp:388
   NgatherGlobal = Ngather;     // ogsSetup.cpp:391-392 at size 1
-  std::vector<dlong> row((size_t)G);
-  {
-    dlong cp = 0, cn = NlocalP;
-    for (dlong g = 0; g < G; ++g) row[g] = (gpos[g] > 0) ? cp++ : cn++;
-  }
 
   gatherLocal = std::make_shared<ogsOperator_t>();
   gatherHalo = std::make_shared<ogsOperator_t>();  // empty at one rank
@@ -152,51 +72,45 @@ void ogs_t::Setup(const dlong _N, hlong *ids, const Kind _kind, const Method met
   gatherHalo->rowStartsN = gatherHalo->rowStartsT;
   gatherHalo->aliased = true;
 
-  if (kind == Halo) {
-    // LocalHaloSetup (ogsSetup.cpp:779-847) keeps only shared nodes, and at
-    // one rank there are none: both operators are empty.
-    L.rowStartsT = memory<dlong>(1, 0);
+  const bool onDevice = d.rsT != nullptr;
+  if (kind == Halo || !onDevice) {
+    // Halo: LocalHaloSetup (ogsSetup.cpp:779-847) keeps only shared nodes,
+    // and at one rank there are none.  Otherwise no node has a nonzero id.
+    L.rowStartsT = memory<dlong>((size_t)(kind == Halo ? 0 : G) + 1, 0);
     L.rowStartsN = L.rowStartsT;
     L.aliased = true;
   } else {
-    const dlong NrowsT = NlocalT;
+    const dlong NrowsT = G;
     L.NrowsT = NrowsT;
     L.NrowsN = NlocalP;
-    L.rowStartsT = memory<dlong>((size_t)NrowsT + 1, 0);
-    for (dlong g = 0; g < G; ++g) L.rowStartsT[(size_t)row[g] + 1] = gsize[g];
-    for (dlong r = 0; r < NrowsT; ++r) L.rowStartsT[(size_t)r + 1] += L.rowStartsT[r];
-    L.nnzT = L.rowStartsT[NrowsT];
-    L.colIdsT = memory<dlong>((size_t)L.nnzT);
-    std::vector<dlong> fillT(L.rowStartsT.ptr(), L.rowStartsT.ptr() + NrowsT);
-    if (kind == Unsigned) {
-      L.aliased = true;
-      L.rowStartsN = L.rowStartsT;
-      L.colIdsN = L.colIdsT;
-      L.nnzN = L.nnzT;
-      for (dlong n = 0; n < N; ++n)
-        if (grp[n] >= 0) L.colIdsT[fillT[row[grp[n]]]++] = n;
+    L.nnzT = d.nnzT;
+    L.nnzN = d.nnzN;
+    L.o_rowStartsT = deviceMemory<dlong>::adopt(d.rsT, (size_t)NrowsT + 1);
+    L.o_colIdsT = deviceMemory<dlong>::adopt(d.ciT, (size_t)L.nnzT);
+    L.aliased = kind == Unsigned;
+    if (L.aliased) {
+      L.o_rowStartsN = L.o_rowStartsT;
+      L.o_colIdsN = L.o_colIdsT;
     } else {
-      L.rowStartsN = memory<dlong>((size_t)NrowsT + 1, 0);
-      for (dlong n = 0; n < N; ++n)
-        if (grp[n] >= 0 && ids[n] > 0) L.rowStartsN[(size_t)row[grp[n]] + 1]++;
-      for (dlong r = 0; r < NrowsT; ++r) L.rowStartsN[(size_t)r + 1] += L.rowStartsN[r];
-      L.nnzN = L.rowStartsN[NrowsT];
-      L.colIdsN = memory<dlong>((size_t)L.nnzN);
-      std::vector<dlong> fillN(L.rowStartsN.ptr(), L.rowStartsN.ptr() + NrowsT);
-      for (dlong n = 0; n < N; ++n) {
-        const dlong g = grp[n];
-        if (g < 0) continue;
-        const dlong r = row[g];
-        L.colIdsT[fillT[r]++] = n;
-        if (ids[n] > 0) L.colIdsN[fillN[r]++] = n;
-      }
+      L.o_rowStartsN = deviceMemory<dlong>::adopt(d.rsN, (size_t)NrowsT + 1);
+      L.o_colIdsN = deviceMemory<dlong>::adopt(d.ciN, (size_t)L.nnzN);
     }
+    // the operator's host row starts (row blocks); colIds come down on demand
+    auto down = [&](memory<dlong> &h, const deviceMemory<dlong> &dv, size_t n) {
+      h = memory<dlong>(n);
+      if (n) LIBP_HIP_CHECK(hipMemcpyAsync(h.ptr(), dv.ptr(), n * sizeof(dlong), hipMemcpyDeviceToHost,
+                                           platform.stream));
+    };
+    down(L.rowStartsT, L.o_rowStartsT, (size_t)NrowsT + 1);
+    if (L.aliased) L.rowStartsN = L.rowStartsT;
+    else down(L.rowStartsN, L.o_rowStartsN, (size_t)NrowsT + 1);
+    platform.finish();
   }
-  grp.clear();
 
   L.setupRowBlocks();             // ogsSetup.cpp:681, 774
   gatherHalo->setupRowBlocks();   // ogsSetup.cpp:682, 775
-  L.upload(platform);
+  if (L.o_rowStartsT.ptr()) L.createPlan(platform);
+  else L.upload(platform);
   gatherHalo->upload(platform);
 
   if (verbose) {
@@ -247,6 +161,25 @@ void ogsOperator_t::setupRowBlocks() {
   blockRowStartsT[bT.size()] = NrowsT;
 }
 
+const memory<dlong> &ogsOperator_t::hostColIdsT() const {
+  if (colIdsT.length() != (size_t)nnzT) {
+    colIdsT = memory<dlong>((size_t)nnzT);
+    if (nnzT) LIBP_HIP_CHECK(hipMemcpy(colIdsT.ptr(), o_colIdsT.ptr(), sizeof(dlong) * (size_t)nnzT,
+                                       hipMemcpyDeviceToHost));
+  }
+  return colIdsT;
+}
+
+const memory<dlong> &ogsOperator_t::hostColIdsN() const {
+  if (aliased) return hostColIdsT();
+  if (colIdsN.length() != (size_t)nnzN) {
+    colIdsN = memory<dlong>((size_t)nnzN);
+    if (nnzN) LIBP_HIP_CHECK(hipMemcpy(colIdsN.ptr(), o_colIdsN.ptr(), sizeof(dlong) * (size_t)nnzN,
+                                       hipMemcpyDeviceToHost));
+  }
+  return colIdsN;
+}
+
 void ogsOperator_t::upload(const platform_t &platform) {
   o_rowStartsT = platform.malloc<dlong>((size_t)NrowsT + 1, rowStartsT.ptr());
   o_colIdsT = platform.malloc<dlong>((size_t)nnzT, colIdsT.ptr());
@@ -257,6 +190,10 @@ void ogsOperator_t::upload(const platform_t &platform) {
     o_rowStartsN = platform.malloc<dlong>((size_t)NrowsT + 1, rowStartsN.ptr());
     o_colIdsN = platform.malloc<dlong>((size_t)nnzN, colIdsN.ptr());
   }
+  createPlan(platform);
+}
+
+void ogsOperator_t::createPlan(const platform_t &platform) {
   if (plan_) { ogs_hip_plan_destroy(plan_); plan_ = nullptr; }
   const int rc = ogs_hip_plan_create(&plan_, Ncols, NrowsN, NrowsT, o_rowStartsN.ptr(), o_colIdsN.ptr(),
                                      o_rowStartsT.ptr(), o_colIdsT.ptr(), platform.stream);
