@@ -168,8 +168,10 @@ typedef struct ogs_handle_s *ogs_handle_t;
  * ids is a host array of N global ids (0 = ignored, negative = flagged);
  * with unique=1 the chosen signs are written back into ids, as the
  * reference does (ogsSetup.cpp:139-140).  The sole positive member of each
- * group is the member with the smallest local index.  Builds the CSR on the
- * host, uploads it, and creates the operator plan. */
+ * group is the member with the smallest local index.  |id| must fit an
+ * int64 (INT64_MIN is no valid id: the reference takes its abs()).  Builds
+ * the CSR on the device (the host keeps the row starts) and creates the
+ * operator plan. */
 int ogs_setup(ogs_handle_t *h, int32_t N, int64_t *ids, int kind, int method,
               int unique, int verbose);
 int ogs_free(ogs_handle_t h);
