@@ -126,6 +126,30 @@ void ogs_t::Scatter(memory<T> v, const memory<T> gv, const int k, const Transpos
   platform.finish();
 }
 
+// ---- SetupGlobalToLocalMapping (ogsSetup.cpp:867-886) ----
+// Scatter (NoTrans) of the row indices 0..NlocalT-1 into an array of -1s:
+// each node receives its row; the halo operator has no rows at one rank.
+void ogs_t::SetupGlobalToLocalMapping(memory<dlong> GlobalToLocal) {
+  LIBP_ABORT("GlobalToLocal shorter than N", GlobalToLocal.length() < (size_t)N, OGS_E_ARG);
+  SetupGlobalToLocalMapping(GlobalToLocal.ptr());
+}
+
+void ogs_t::SetupGlobalToLocalMapping(dlong *GlobalToLocal) {
+  LIBP_ABORT("ogs handle is not set up.", !gatherLocal, OGS_E_ARG);
+  LIBP_ABORT("Null GlobalToLocal", N > 0 && !GlobalToLocal, OGS_E_ARG);
+  if (N == 0) return;
+  std::vector<dlong> rows((size_t)NlocalT + NhaloT);
+  for (size_t n = 0; n < rows.size(); ++n) rows[n] = (dlong)n;
+  std::vector<dlong> minus((size_t)N, -1);
+  deviceMemory<dlong> o_map = platform.malloc<dlong>((size_t)N, minus.data());
+  deviceMemory<dlong> o_rows = platform.malloc<dlong>(rows.size(), rows.data());
+  gatherLocal->Scatter(o_map, o_rows, 1, NoTrans, platform.stream);
+  gatherHalo->Scatter(o_map, o_rows + (size_t)NlocalT, 1, NoTrans, platform.stream);
+  LIBP_HIP_CHECK(hipMemcpyAsync(GlobalToLocal, o_map.ptr(), sizeof(dlong) * (size_t)N, hipMemcpyDeviceToHost,
+                                platform.stream));
+  platform.finish();
+}
+
 #define OGS_INSTANTIATE(T)                                                                           \
   template void ogs_t::GatherScatter(deviceMemory<T>, const int, const Op, const Transpose);        \
   template void ogs_t::GatherScatter(memory<T>, const int, const Op, const Transpose);              \
@@ -248,6 +272,11 @@ int ogs_get_csr(ogs_handle_t h, int32_t *rowStartsN, int32_t *colIdsN, int32_t *
 }
 
 ogs_hip_plan_t ogs_get_plan(ogs_handle_t h) { return h ? h->ogs.local().plan() : nullptr; }
+
+int ogs_setup_global_to_local(ogs_handle_t h, int32_t *GlobalToLocal) {
+  if (!h) return OGS_E_ARG;
+  return guarded([&] { h->ogs.SetupGlobalToLocalMapping(GlobalToLocal); });
+}
 
 int ogs_gather_scatter(ogs_handle_t h, int type, int op, int trans, int32_t K, void *d_v, void *stream) {
   if (!h || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;

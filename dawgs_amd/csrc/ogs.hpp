@@ -238,6 +238,11 @@ class ogs_t {
   template <typename T> void ScatterFinish(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans);
   template <typename T> void Scatter(memory<T> v, const memory<T> gv, const int k, const Transpose trans);
 
+  // ogsSetup.cpp:867-886: GlobalToLocal[n] = the gathered index (row) of node
+  // n, -1 where no row holds it (id 0); GlobalToLocal has N entries.
+  void SetupGlobalToLocalMapping(memory<dlong> GlobalToLocal);
+  void SetupGlobalToLocalMapping(dlong *GlobalToLocal);
+
   const ogsOperator_t &local() const { return *gatherLocal; }
   void AssertGatherDefined() const;  // ogsSetup.cpp:861-864
 

@@ -208,6 +208,13 @@ class ogs_t:
               "ogs_get_csr")
         return a
 
+    def SetupGlobalToLocalMapping(self) -> np.ndarray:
+        """ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:867-886): each node's
+        gathered index (row), -1 for nodes with id 0."""
+        out = np.empty(self.N, dtype=np.int32)
+        check(lib().ogs_setup_global_to_local(self.h, out.ctypes.data_as(C.c_void_p)), "ogs_setup_global_to_local")
+        return out
+
     def GatherScatter(self, v, k: int, op: int, trans: int, stream=None, type=None) -> None:
         check(lib().ogs_gather_scatter(self.h, type_of(v, type), op, trans, k, _dev_ptr(v), _stream(stream)),
               "ogs_gather_scatter")

@@ -195,6 +195,11 @@ int ogs_get_csr(ogs_handle_t h, int32_t *rowStartsN, int32_t *colIdsN,
 ogs_hip_plan_t ogs_get_plan(ogs_handle_t h);
 
 /* ogs_t::GatherScatter on a device buffer (ogs.cpp:39-123). */
+/* ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:867-886): GlobalToLocal (a
+ * host array of N entries) receives each node's gathered index (its row),
+ * -1 for nodes with id 0.  Synchronises. */
+int ogs_setup_global_to_local(ogs_handle_t h, int32_t *GlobalToLocal);
+
 int ogs_gather_scatter(ogs_handle_t h, int type, int op, int trans, int32_t K,
                        void *d_v, void *stream);
 /* ogs_t::Gather on device buffers (ogs.cpp:202-298); needs a gather-defined

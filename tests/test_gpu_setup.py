@@ -63,3 +63,25 @@ def test_extreme_magnitudes():
         assert g.NlocalT == o.NlocalT == 4
         for name in ("rowStartsT", "colIdsT", "rowStartsN", "colIdsN"):
             assert np.array_equal(c[name], getattr(o, name)), (kind, name)
+
+
+@pytest.mark.parametrize("kind,unique", [(O.UNSIGNED, False), (O.SIGNED, False), (O.SIGNED, True)])
+def test_global_to_local_mapping(kind, unique):
+    """SetupGlobalToLocalMapping (ogsSetup.cpp:867-886): a NoTrans Scatter of
+    the row indices, so every member of row r maps to r and id 0 to -1."""
+    import dawgs_amd as D
+    rng = np.random.default_rng(3)
+    ids = O.make_ids(5, 3, 4, 2).copy()
+    ids[rng.random(len(ids)) < 0.1] = 0
+    if kind == O.SIGNED:
+        neg = rng.random(len(ids)) < 0.3
+        ids[neg] = -ids[neg]
+    o = O.OracleOgs(ids, kind, unique)
+    g = D.ogs_t()
+    g.Setup(len(ids), ids.copy(), kind, D.Auto, unique, False)
+    want = np.full(len(ids), -1, dtype=np.int32)
+    for r in range(o.NlocalT):
+        want[o.colIdsT[o.rowStartsT[r]:o.rowStartsT[r + 1]]] = r
+    got = g.SetupGlobalToLocalMapping()
+    assert np.array_equal(got, want)
+    assert np.all(got[ids == 0] == -1) and np.all(got[ids != 0] >= 0)
