@@ -12,20 +12,31 @@
 //    staging never overflows, and the cut is computed on the device by
 //    binary search (no host pass over the CSR).  Blocks are contiguous row
 //    ranges, so each block's column set is a compact window of v.
-//  * Loads.  Index streams (colIds, row starts) are read coalesced, each lane
-//    holding a fixed number of slots; all slots' loads are issued before the
-//    first wait, so a wave has 8 independent gathers in flight.
-//  * Reduction.  Gathered values are staged in LDS in CSR order; one lane per
-//    row reduces its row sequentially, which keeps fp Add bit-exact with the
-//    host loop.  Integer Add/Mul wrap (unsigned arithmetic); Min/Max are the
+//  * Packs.  For the partitions a plan serves, each block's list entries are
+//    sorted by column at plan creation and stored as 32-bit words (column
+//    segment and offset, row-head bit, CSR position) at a fixed stride per
+//    block, so a wave's gathers touch few cache lines and a kernel loads a
+//    block's words from the block id alone.
+//  * Loads.  Index words are read coalesced, each lane holding a fixed number
+//    of slots; all slots' loads are issued before the first wait, and the
+//    gathers run unpredicated, so a wave keeps all of them in flight.
+//  * Reduction.  Gathered values are staged in LDS in CSR order.  With one
+//    list (Sym, or an Unsigned operator) the row heads become a bit mask and
+//    one lane per (row head, field chunk) folds the row in CSR order; with
+//    two lists one lane per row reduces the gather list and writes the
+//    scatter list.  Either way fp Add is bit-exact with the host loop.
+//    Integer Add/Mul wrap (unsigned arithmetic); Min/Max are the
 //    reference's compare-selects, so NaN inputs are skipped like on the host.
+//    The precision-form storage types (bfloat16, float with compensated Add)
+//    accumulate in float with a two-sum running error.
 //  * XCDs.  Workgroups are dispatched round-robin over the 8 XCDs; block ids
 //    are remapped so that each XCD walks one contiguous eighth of the rows
 //    and neighbouring blocks share that XCD's L2.
-//  * K fields.  Fields are interleaved (v[k + id*K]); all K fields of a block
-//    are done in one pass over its indices when they fit in LDS, else in
-//    field chunks, so indices are read once per block rather than once per
-//    field as in the reference (ogsKernels.okl:41, @outer(1) over K).
+//  * K fields.  Fields are interleaved (v[k + id*K]).  For K in {2, 3, 4, 8,
+//    16} a work item is up to 4 consecutive fields of one entry (one index
+//    word feeds contiguous value loads); other K run in field chunks.  The
+//    indices are read once per block rather than once per field as in the
+//    reference (ogsKernels.okl:41, @outer(1) over K).
 //
 // No MFMA: nothing here is a dense contraction.  No fast-math: the build
 // uses -ffp-contract=off and keeps IEEE semantics.
