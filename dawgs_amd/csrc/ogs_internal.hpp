@@ -11,7 +11,7 @@ namespace ogs_hip {
 constexpr int kBlock = 256;                                     // lanes per workgroup (4 waves of 64)
 constexpr int kCap = 2048;                                      // values staged in LDS per list per pass
 constexpr int kSlots = kCap / kBlock;                           // staged values per lane
-constexpr int kWMax = kCap;                                     // rows per block never exceed the widest W
+constexpr int kWMax = kCap;                                     // rows per block: <= W (merge path) or <= nonzeros <= kCap
 constexpr int kRowSlots = (kWMax + 1 + kBlock - 1) / kBlock;    // row starts per lane
 constexpr int kNumXcd = 8;                                      // MI355X: 8 XCDs x 32 CUs
 constexpr int kMaxParts = 7;                                    // partition widths kCap>>0 .. kCap>>6
@@ -31,15 +31,16 @@ constexpr int kPackSegs = 4;
 constexpr int kPackParts = 5;  // T-list packs for partitions 0..4 (K up to 16 in one pass)
 static_assert(kCap <= (1 << kPackPosBits), "CSR positions of a block must fit the pack word");
 
-// One merge-path row partition: nblocks+1 entries {row, rowStartsT[row],
+// One row partition: nblocks+1 entries {row, rowStartsT[row],
 // rowStartsN[row], 0}; block b covers rows [part[b].x, part[b+1].x).
 struct Partition {
   int4 *d = nullptr;
   int32_t nblocks = 0;
-  int32_t W = 0;        // keys (rows + nonzeros) per block
+  int32_t W = 0;        // keys per block: nonzeros, plus rows when rowsInKey
+  bool rowsInKey = true;  // merge path (a list with empty rows)
   int32_t maxT = 0;     // largest T-list nonzero count of a block
   int32_t maxN = 0;     // largest N-list nonzero count of a block
-  int32_t maxRows = 0;  // largest row count of a block (<= W)
+  int32_t maxRows = 0;  // largest row count of a block
 };
 
 // A list's entries in the blocks of one partition, each block's entries
@@ -69,11 +70,15 @@ struct ogs_hip_plan_s {
   int nparts = 0;
   ogs_hip::Partition partT[ogs_hip::kMaxParts];  // cut on T, rows [0, NrowsT)
   ogs_hip::Partition partN[ogs_hip::kMaxParts];  // cut on N, rows [0, NrowsN)
+  // T-list partitions of the scatter kernels: merge path (measured faster for
+  // them than blocks cut on entries); copies of partT when those are merge path
+  ogs_hip::Partition partS[ogs_hip::kMaxParts];
   // column-sorted packs: the T list over partT[0..kPackParts) (K=1 and the
   // K-field Sym kernels), the N list over partT[0] and over partN[0] (K=1)
   ogs_hip::Pack packT[ogs_hip::kPackParts];
   ogs_hip::Pack packNT;
   ogs_hip::Pack packNN;
+  ogs_hip::Pack packS;  // the T list over partS[0] (a copy of packT[0] when partS is partT)
 };
 
 namespace ogs_hip {

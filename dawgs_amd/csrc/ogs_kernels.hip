@@ -1213,8 +1213,8 @@ __global__ __launch_bounds__(kBlock) void k_extract(int64_t total, int K, const 
 // Plan-time kernels.
 
 // Longest row of a CSR and a validity check of its row starts and column ids.
-// out[0] = max row length, out[1] = error flags (1: decreasing row start,
-// 2: column id out of [0, ncols)).
+// out[0] = max row length, out[1] = flags (1: decreasing row start, 2: column
+// id out of [0, ncols) -- both errors; 4: some row is empty).
 __global__ __launch_bounds__(kBlock) void k_scan_csr(const int32_t *__restrict__ rs, int nrows,
                                                      const int32_t *__restrict__ ci, int nnz, int ncols,
                                                      int32_t *__restrict__ out) {
@@ -1226,6 +1226,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_csr(const int32_t *__restrict__
   for (int r = blockIdx.x * kBlock + threadIdx.x; r < nrows; r += stride) {
     const int len = rs[r + 1] - rs[r];
     if (len < 0) bad |= 1;
+    if (len == 0) bad |= 4;
     lmax = max(lmax, len);
   }
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nnz; i += stride) {
@@ -1241,11 +1242,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_csr(const int32_t *__restrict__
   }
 }
 
-// Merge-path partition: entry b = first row r with key(r) >= b*W, where
-// key(r) = rowStartsX[r] + r (X = the list the partition is cut on).
-// Entry nblocks = nrows.  Entries carry both lists' starts.
+// Row partition: entry b = first row r with key(r) >= b*W, where key(r) =
+// rowStartsX[r] (X = the list the partition is cut on), plus r when
+// rowsInKey (merge path).  Entry nblocks = nrows.  Entries carry both lists'
+// starts.  Cut on entries alone, a list without empty rows gets blocks of
+// about W entries and never more rows than entries; a list with empty rows
+// needs the merge path, which bounds a block's rows and entries together.
 __global__ __launch_bounds__(kBlock) void k_partition(const int32_t *__restrict__ rsT, const int32_t *__restrict__ rsN,
-                                                      int cutOnN, int nrows, int W, int nblocks,
+                                                      int cutOnN, int rowsInKey, int nrows, int W, int nblocks,
                                                       int4 *__restrict__ part) {
   const int b = blockIdx.x * kBlock + threadIdx.x;
   if (b > nblocks) return;
@@ -1258,7 +1262,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(const int32_t *__restrict_
     int lo = 0, hi = nrows;  // answer in [0, nrows]
     while (lo < hi) {
       const int mid = lo + (hi - lo) / 2;
-      if ((int64_t)(rs[mid] - rs[0]) + mid < target) lo = mid + 1;
+      if ((int64_t)(rs[mid] - rs[0]) + (rowsInKey ? mid : 0) < target) lo = mid + 1;
       else hi = mid;
     }
     row = lo;
@@ -1448,7 +1452,9 @@ static ListView list_view(const Plan *p, bool listN, const Partition *pt, bool p
   L.pk = nullptr;
   L.base = nullptr;
   const Pack *pk = nullptr;
-  if (partIsN) {
+  if (pt == &p->partS[0]) {
+    if (!listN) pk = &p->packS;
+  } else if (partIsN) {
     if (pt == &p->partN[0] && listN) pk = &p->packNN;
   } else {
     const int j = (int)(pt - p->partT);
@@ -1565,10 +1571,12 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   const ListView G = list_view(p, gListN, pt, false), S = list_view(p, sListN, pt, false);
   // one list (rows [0, NrowsT)) and a field count with a compile-time
   // shape: the entry-major kernel on the widest partition whose blocks fit
-  // that shape.  For K = 1 partition 0 always fits.
+  // that shape (for K = 1 partition 0 always fits), one step narrower when
+  // that leaves fewer than 16 workgroups per CU to spread.
   if (sym && p->nnzT == 0) return OGS_SUCCESS;  // no entries: nothing to write
   if (sym && vec_k(K)) {
     const Partition *vp = vec_partition<T>(p, K, 0);
+    if (vp && vp->nblocks < 16 * p->cus && vp + 1 < p->partT + p->nparts && vp + 1 < p->partT + kPackParts) ++vp;
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
 #define OGS_GS_LAUNCH(OPV)                                                                                      \
@@ -1632,7 +1640,7 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
   const bool partN = useN && !p->aliased;
-  const Partition *parts = partN ? p->partN : p->partT;
+  const Partition *parts = partN ? p->partN : p->partS;
   int kc = 0;
   const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
   if (!pt) return OGS_E_ROW_TOO_LONG;
@@ -1689,16 +1697,18 @@ static int scan_csr(const int32_t *rs, int nrows, const int32_t *ci, int nnz, in
   return OGS_SUCCESS;
 }
 
-static int build_partition(Plan *p, bool cutOnN, int nrows, int W, int32_t *scratch, hipStream_t st, Partition *out) {
-  const int64_t keys = (int64_t)(cutOnN ? p->nnzN : p->nnzT) + nrows;
+static int build_partition(Plan *p, bool cutOnN, bool rowsInKey, int nrows, int W, int32_t *scratch, hipStream_t st,
+                           Partition *out) {
+  const int64_t keys = (int64_t)(cutOnN ? p->nnzN : p->nnzT) + (rowsInKey ? nrows : 0);
   const int64_t nb = (keys + W - 1) / W;
   if (nb > INT32_MAX - 1) return OGS_E_ARG;
   out->W = W;
+  out->rowsInKey = rowsInKey;
   out->nblocks = (int)nb;
   hipError_t e = hipMalloc(&out->d, sizeof(int4) * (size_t)(nb + 1));
   if (e != hipSuccess) return (int)e;
   hipLaunchKernelGGL(k_partition, dim3(grid_for(nb + 1, 1 << 30)), dim3(kBlock), 0, st, p->rsT, p->rsN,
-                     cutOnN ? 1 : 0, nrows, W, (int)nb, out->d);
+                     cutOnN ? 1 : 0, rowsInKey ? 1 : 0, nrows, W, (int)nb, out->d);
   e = hipMemsetAsync(scratch, 0, 3 * sizeof(int32_t), st);
   if (e != hipSuccess) return (int)e;
   hipLaunchKernelGGL(k_part_max, dim3(grid_for(nb, 2048)), dim3(kBlock), 0, st, out->d, (int)nb, scratch);
@@ -1749,10 +1759,12 @@ static void free_plan(Plan *p) {
   const bool nn = p->packNN.pk && p->packNN.pk != p->packT[0].pk;
   if (nt) free_pack(p->packNT);
   if (nn) free_pack(p->packNN);
+  if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
   for (int j = 0; j < p->nparts; ++j) {
     if (p->partT[j].d) (void)hipFree(p->partT[j].d);
     if (p->partN[j].d && p->partN[j].d != p->partT[j].d) (void)hipFree(p->partN[j].d);
+    if (p->partS[j].d && p->partS[j].d != p->partT[j].d) (void)hipFree(p->partS[j].d);
   }
   delete p;
 }
@@ -1820,17 +1832,27 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     if (p->aliased) { p->maxRowN = p->maxRowT; }
     else rc = scan_csr(p->rsN, NrowsT, p->ciN, p->nnzN, Ncols, scratch, st, &p->maxRowN, &fl2);
   }
-  if (!rc && (flags | fl2)) rc = OGS_E_CSR;
+  if (!rc && ((flags | fl2) & 3)) rc = OGS_E_CSR;
+  // T lists without empty rows (every operator that Setup builds) are cut on
+  // entries alone: measured faster than merge-path blocks of the same width
+  // for GatherScatter and Gather (more entries per workgroup, fewer blocks).
+  // The scatter kernels keep merge-path partitions (partS), measured faster
+  // for them.  N lists may have empty rows and are cut by the merge path.
+  const bool cutTOnEntries = (flags & 4) == 0;
   const int lmax = p->maxRowT > p->maxRowN ? p->maxRowT : p->maxRowN;
   if (!rc && lmax > kCap) rc = OGS_E_ROW_TOO_LONG;
   // Partition widths: block nonzeros stay <= kCap >> j.
   for (int j = 0; !rc && j < kMaxParts && NrowsT > 0; ++j) {
     const int W = (kCap >> j) - (lmax > 0 ? lmax - 1 : 0);
     if (W < 1) break;
-    rc = build_partition(p, false, NrowsT, W, scratch, st, &p->partT[j]);
+    rc = build_partition(p, false, !cutTOnEntries, NrowsT, W, scratch, st, &p->partT[j]);
     if (!rc) {
       if (p->aliased) p->partN[j] = p->partT[j];
-      else rc = build_partition(p, true, NrowsN, W, scratch, st, &p->partN[j]);
+      else rc = build_partition(p, true, true, NrowsN, W, scratch, st, &p->partN[j]);
+    }
+    if (!rc) {
+      if (cutTOnEntries) rc = build_partition(p, false, true, NrowsT, W, scratch, st, &p->partS[j]);
+      else p->partS[j] = p->partT[j];
     }
     p->nparts = j + 1;
   }
@@ -1844,6 +1866,10 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     } else if (!rc) {
       rc = build_pack(p->partT[0], NrowsT, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNT);
       if (!rc) rc = build_pack(p->partN[0], NrowsN, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNN);
+    }
+    if (!rc) {
+      if (cutTOnEntries) rc = build_pack(p->partS[0], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packS);
+      else p->packS = p->packT[0];
     }
   }
   (void)hipFree(scratch);
@@ -2152,6 +2178,8 @@ static void free_batch(ogs_hip_batch_s *b) {
 }
 
 // Times a configuration on scratch buffers: 1 warm-up, 3 timed launches.
+constexpr int kTuneLaunches = 5;  // timed launches per candidate and pass
+
 static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
                        const BatchConfig &c, int cus, hipStream_t st, float *us) {
   BatchTables t;
@@ -2164,7 +2192,7 @@ static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t>
   if (!rc) rc = (int)hipEventCreate(&e1);
   if (!rc) rc = launch_batch(type, op, dt, cus, c.fused, st);
   if (!rc) rc = (int)hipEventRecord(e0, st);
-  for (int i = 0; i < 3 && !rc; ++i) rc = launch_batch(type, op, dt, cus, c.fused, st);
+  for (int i = 0; i < kTuneLaunches && !rc; ++i) rc = launch_batch(type, op, dt, cus, c.fused, st);
   if (!rc) rc = (int)hipEventRecord(e1, st);
   if (!rc) rc = (int)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -2172,7 +2200,7 @@ static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t>
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   dt.release();
-  *us = ms * 1e3f / 3.f;
+  *us = ms * 1e3f / kTuneLaunches;
   return rc;
 }
 
@@ -2216,20 +2244,26 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
       rc = (int)hipMalloc(&scratch[i], bytes);
       if (!rc) rc = (int)hipMemcpyAsync(scratch[i], in[i].d_v, bytes, hipMemcpyDeviceToDevice, st);
     }
+    // two passes, each candidate keeping its faster time: the first launches
+    // after a setup run slow, and one pass of a few launches can mis-rank
+    std::vector<BatchConfig> cands;
+    for (int fused = 0; fused <= 1; ++fused)
+      for (int narrow = 0; narrow <= (anyKv ? 1 : 0); ++narrow)
+        for (int order = 0; order <= 1; ++order) cands.push_back({narrow, order, fused});
+    std::vector<float> times(cands.size(), 0.f);
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+      for (size_t i = 0; i < cands.size() && !rc; ++i) {
+        float us = 0.f;
+        rc = time_config(type, op, in, scratch, cands[i], cus, st, &us);
+        if (!rc && (times[i] == 0.f || us < times[i])) times[i] = us;
+      }
+    }
+    b->candidates = (int)cands.size();
     float bestUs = 0.f;
-    for (int fused = 0; fused <= 1; ++fused) {
-      for (int narrow = 0; narrow <= (anyKv ? 1 : 0); ++narrow) {
-        for (int order = 0; order <= 1; ++order) {
-          if (rc) break;
-          const BatchConfig c = {narrow, order, fused};
-          float us = 0.f;
-          rc = time_config(type, op, in, scratch, c, cus, st, &us);
-          ++b->candidates;
-          if (!rc && (bestUs == 0.f || us < bestUs)) {
-            bestUs = us;
-            best = c;
-          }
-        }
+    for (size_t i = 0; i < cands.size() && !rc; ++i) {
+      if (bestUs == 0.f || times[i] < bestUs) {
+        bestUs = times[i];
+        best = cands[i];
       }
     }
     for (void *q : scratch)

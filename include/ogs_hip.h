@@ -135,7 +135,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
  * any other K.  Buffers are bound at creation.  With tune != 0 the creation
  * times candidate configurations (one node or one per class, block width of
  * K > 1 classes, item order) on scratch copies of the
- * buffers and keeps the fastest; the buffers themselves are not touched.
+ * buffers (best of two passes each) and keeps the fastest; the buffers
+ * themselves are not touched.
  * Creation allocates and synchronises; ogs_hip_batch_run only launches the
  * graph.  Concurrent runs of one batch race on its buffers. */
 typedef struct {

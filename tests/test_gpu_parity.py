@@ -384,6 +384,54 @@ def test_hand_built_operator_long_rows(torch, D):
             assert same(to_host(torch, d), ws), (K, trans)
 
 
+def test_partition_cut_follows_empty_rows(torch, D):
+    """T lists without empty rows (every Setup-built operator) are cut into
+    blocks of W entries; a T list with empty rows -- here scattered ones and a
+    run of 5000 -- is cut on entries + rows (merge path), which bounds a
+    block's rows too.  Both must give the oracle's results."""
+    import math
+    g = D.ogs_t()
+    ids = O.make_ids(6, 5, 4, 3)
+    g.Setup(len(ids), ids.copy(), D.Unsigned)
+    i = g.plan.info()
+    assert i["nblocksK1"] == math.ceil(i["nnzT"] / i["widthK1"]), i
+    rng = np.random.default_rng(17)
+    Ncols = 30_000
+    lens = rng.integers(0, 9, 9000)
+    lens[3000:8000] = 0
+    lens = lens[: np.searchsorted(np.cumsum(lens), Ncols)]
+    NrowsT = len(lens)
+    nnzT = int(lens.sum())
+    cols = rng.permutation(Ncols)[:nnzT].astype(np.int32)
+    rsT = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ciT = np.concatenate([np.sort(cols[rsT[r]:rsT[r + 1]]) for r in range(NrowsT)]).astype(np.int32)
+    keep = rng.random(nnzT) < 0.7
+    rows = np.repeat(np.arange(NrowsT), lens)
+    ciN = ciT[keep]
+    rsN = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=NrowsT))]).astype(np.int32)
+    ref = O.OracleCsr(rsT, ciT, rsN, ciN, NrowsT, Ncols)
+    plan = D.Plan(Ncols, NrowsT, NrowsT, to_dev(torch, rsN), to_dev(torch, ciN), to_dev(torch, rsT), to_dev(torch, ciT))
+    i = plan.info()
+    assert i["nblocksK1"] == math.ceil((nnzT + NrowsT) / i["widthK1"]), i
+    for K in (1, 2, 3, 8):
+        v = rng.uniform(-1, 1, Ncols * K)
+        for trans in TRANS:
+            for op in (O.ADD, O.MAX):
+                want = ref.gather_scatter(v.copy(), K, op, trans)
+                d = to_dev(torch, v)
+                plan.GatherScatter(d, K, op, trans)
+                assert same(to_host(torch, d), want), (K, trans, op)
+            wg = ref.gather(v, K, O.ADD, trans)
+            gv = torch.zeros(NrowsT * K, dtype=torch.float64, device="cuda")
+            plan.Gather(gv, to_dev(torch, v), K, O.ADD, trans)
+            assert same(to_host(torch, gv)[: len(wg)], wg), (K, trans)
+            gvals = rng.uniform(-1, 1, NrowsT * K)
+            ws = ref.scatter(v.copy(), gvals, K, trans)
+            d = to_dev(torch, v)
+            plan.Scatter(d, to_dev(torch, gvals), K, trans)
+            assert same(to_host(torch, d), ws), (K, trans)
+
+
 def test_unsortable_blocks_fall_back_to_csr_order(torch, D):
     """Columns scattered over 4M entries: a block's sorted columns need more
     than 4 runs of 2^19, so the plan leaves such blocks in CSR order.  Both
@@ -391,10 +439,10 @@ def test_unsortable_blocks_fall_back_to_csr_order(torch, D):
     rng = np.random.default_rng(9)
     Ncols = 4_000_000
     lens = rng.integers(1, 9, 40_000)
-    lens[:2000] = 1  # short rows over a narrow column range: packable blocks
+    lens[:5000] = 1  # short rows over a narrow column range: two whole packable blocks
     nnz = int(lens.sum())
-    narrow = np.sort(rng.choice(np.arange(Ncols - 50_000, Ncols), 2000, replace=False))
-    wide = rng.choice(Ncols - 50_000, nnz - 2000, replace=False)
+    narrow = np.sort(rng.choice(np.arange(Ncols - 50_000, Ncols), 5000, replace=False))
+    wide = rng.choice(Ncols - 50_000, nnz - 5000, replace=False)
     cols = np.concatenate([narrow, wide]).astype(np.int32)
     rsT = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     ciT = np.concatenate([np.sort(cols[rsT[r]:rsT[r + 1]]) for r in range(len(lens))]).astype(np.int32)
