@@ -70,8 +70,9 @@ struct ogs_hip_plan_s {
   int nparts = 0;
   ogs_hip::Partition partT[ogs_hip::kMaxParts];  // cut on T, rows [0, NrowsT)
   ogs_hip::Partition partN[ogs_hip::kMaxParts];  // cut on N, rows [0, NrowsN)
-  // T-list partitions of the scatter kernels: merge path (measured faster for
-  // them than blocks cut on entries); copies of partT when those are merge path
+  // T-list partitions of the scatter kernels: merge path (measured faster
+  // for them than blocks cut on entries); copies of partT when those are
+  // merge path (a T list with empty rows)
   ogs_hip::Partition partS[ogs_hip::kMaxParts];
   // column-sorted packs: the T list over partT[0..kPackParts) (K=1 and the
   // K-field Sym kernels), the N list over partT[0] and over partN[0] (K=1)

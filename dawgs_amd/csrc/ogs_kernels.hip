@@ -1028,6 +1028,89 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
 }
 
+// Scatter, K = 1, of a T list without empty rows over an entry-cut
+// partition: the block's row values are staged in LDS; an entry's row is the
+// number of row heads at or before its CSR position (the head bit mask and
+// per-word prefix counts), and each entry stores its row's value through its
+// pack word (or in CSR order for a block the pack could not encode).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__ part,
+                                                        const int32_t *__restrict__ starts,
+                                                        const int32_t *__restrict__ ids,
+                                                        const uint32_t *__restrict__ pk,
+                                                        const int4 *__restrict__ base, int stride,
+                                                        const T *__restrict__ gq, T *__restrict__ q) {
+  constexpr int SL = kSlots;  // kCap entries and at most kCap rows per block
+  constexpr int NW = kCap / 32;
+  static_assert(NW == 64, "one wave scans the head-mask words");
+  __shared__ T srow[kCap];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[NW + 1];
+  __shared__ int32_t pc[NW];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
+  const VecSrc src = {part, ids, pk, base, q, 0, stride};
+  VecStage<SL> s;
+  VecMeta m;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  vec_meta(src, b, m);
+  vec_index<T, 1>(s, src, b, m);
+  const int n = s.n;
+  if (n == 0) return;
+  const int tid = threadIdx.x;
+  const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;  // nr <= n: no empty rows
+  T x[SL];
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int r = tid + j * kBlock;
+    x[j] = r < nr ? gq[r0 + r] : T{};
+  }
+  const bool packed = s.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < nr) srow[it] = x[j];
+    if (it < n) {
+      if (packed) head[s.w[j] & kPackPosMask] = (uint8_t)((s.w[j] >> kPackHeadBit) & 1u);
+      else head[it] = 0;
+    }
+  }
+  if (!packed) {
+    int rg[kRowSlots];
+    row_words(starts, r0, nr, rg);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRowSlots; ++j) {
+      const int r = tid + j * kBlock;
+      if (r < nr) head[rg[j] - s.g0] = 1;
+    }
+  }
+  __syncthreads();
+  head_mask(head, n, hm);
+  __syncthreads();
+  if (tid < NW) {  // exclusive prefix counts of the heads, word by word
+    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < NW; d <<= 1) {
+      const int y = __shfl_up(incl, d, NW);
+      if (tid >= d) incl += y;
+    }
+    pc[tid] = incl - c;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < n) {
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it;
+      const int wd = ps >> 5;
+      const uint32_t upto = (uint32_t)((2ull << (ps & 31)) - 1u);  // positions 32 wd .. ps
+      const int row = pc[wd] + __popc(hm[wd] & upto) - 1;
+      q[slot_col(s.w[j], s.bb, packed)] = srow[row];
+    }
+  }
+}
+
 // Batched form: one (descriptor, block) item per workgroup.
 template <typename T, int OP, int KT>
 __device__ __forceinline__ void batch_item(const BatchDev *__restrict__ descs, int2 w,
@@ -1640,6 +1723,19 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
   const bool partN = useN && !p->aliased;
+  // K = 1 from a T list without empty rows: rows staged in LDS, each entry's
+  // row found by counting heads, on the merge-path blocks (measured faster
+  // for this kernel than blocks cut on entries, except at degree 1); when
+  // those leave fewer than 16 workgroups per CU, on the entry-cut blocks one
+  // step narrower (more blocks, with a pack)
+  if (K == 1 && !partN && !p->partT[0].rowsInKey && p->nnzT > 0 && p->partS[0].maxT <= kCap) {
+    const Partition *vp = &p->partS[0];
+    if (vp->nblocks < 16 * p->cus && p->nparts > 1) vp = &p->partT[1];
+    const ListView S = list_view(p, false, vp, false);
+    hipLaunchKernelGGL((k_scatter_vec<T>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
+                       S.base, S.stride, static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+    return (int)hipGetLastError();
+  }
   const Partition *parts = partN ? p->partN : p->partS;
   int kc = 0;
   const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);

@@ -269,6 +269,16 @@ def test_headline_48cubed_p7(torch, D):
     d = to_dev(torch, v)
     g.GatherScatter(d, 1, D.Add, D.Sym)
     assert same(to_host(torch, d), want)
+    # plan-level Gather Trans and Scatter NoTrans at full size (the widest partitions)
+    wg = ref.gather(v, 1, O.ADD, O.TRANS)
+    gq = torch.zeros(len(wg), dtype=torch.float64, device="cuda")
+    g.plan.Gather(gq, to_dev(torch, v), 1, O.ADD, O.TRANS)
+    assert same(to_host(torch, gq), wg)
+    base = rng.uniform(-1, 1, N)
+    d = to_dev(torch, base)
+    g.plan.Scatter(d, gq, 1, O.NOTRANS)
+    assert same(to_host(torch, d), ref.scatter(base.copy(), wg, 1, O.NOTRANS))
+    del gq, base
     ones = torch.ones(N, dtype=torch.float64, device="cuda")
     g.GatherScatter(ones, 1, D.Add, D.Sym)
     m = to_host(torch, ones)
