@@ -1028,6 +1028,96 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
 }
 
+// Gather, K = 1, of a T list without empty rows: the GatherScatter body up to
+// the row reduction (values gathered through the column-sorted pack and
+// staged at their CSR positions, rows found from the head bits, so no row
+// starts are read), each row's result stored at its row -- the number of row
+// heads at or before its head's position -- instead of scattered back.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
+                                                       const int32_t *__restrict__ starts,
+                                                       const int32_t *__restrict__ ids,
+                                                       const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       const T *__restrict__ q, T *__restrict__ gq) {
+  using V = VecShape<T, 1>;
+  using L = typename Lds<T>::L;
+  using R = LRed<T, OP>;
+  constexpr int SL = V::SL;
+  constexpr int NW = kCap / 32;
+  static_assert(NW == 64, "one wave scans the head-mask words");
+  static_assert(V::CAP == kCap, "K = 1 blocks hold kCap entries");
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[NW + 1];
+  __shared__ int32_t pc[NW];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
+  const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
+  VecStage<SL> s;
+  VecMeta m;
+  T x[SL][1];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  vec_meta(src, b, m);
+  vec_index<T, 1>(s, src, b, m);
+  vec_gather<T, 1>(s, x);
+  const int n = s.n;
+  if (n == 0) return;
+  const int tid = threadIdx.x;
+  const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;
+  const bool packed = s.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < n) {
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it;
+      val[ps] = Lds<T>::in(x[j][0]);
+      head[ps] = packed ? (uint8_t)((s.w[j] >> kPackHeadBit) & 1u) : (uint8_t)0;
+    }
+  }
+  if (!packed) {
+    int rg[kRowSlots];
+    row_words(starts, r0, nr, rg);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRowSlots; ++j) {
+      const int r = tid + j * kBlock;
+      if (r < nr) head[rg[j] - s.g0] = 1;
+    }
+  }
+  __syncthreads();
+  head_mask(head, n, hm);
+  __syncthreads();
+  if (tid < NW) {  // exclusive prefix counts of the heads, word by word
+    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < NW; d <<= 1) {
+      const int y = __shfl_up(incl, d, NW);
+      if (tid >= d) incl += y;
+    }
+    pc[tid] = incl - c;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += kBlock) {
+    const int wd = i >> 5;
+    const uint32_t bits = hm[wd];
+    if (!((bits >> (i & 31)) & 1u)) continue;
+    const int len = row_len(hm, i);
+    typename R::A red = R::init();
+    int k = 0;
+    for (; k + 4 <= len; k += 4) {  // CSR order, members read four at a time
+      L y[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) y[u] = val[i + k + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) R::apply(red, y[u]);
+    }
+    for (; k < len; ++k) R::apply(red, val[i + k]);
+    const int row = pc[wd] + __popc(bits & (uint32_t)((2ull << (i & 31)) - 1u)) - 1;
+    gq[r0 + row] = Lds<T>::out(R::fin(red));
+  }
+}
+
 // Scatter, K = 1, of a T list without empty rows over an entry-cut
 // partition: the block's row values are staged in LDS; an entry's row is the
 // number of row heads at or before its CSR position (the head bit mask and
@@ -1688,6 +1778,32 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
   const bool useN = (trans == OGS_NOTRANS);
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  // K = 1 from a T list without empty rows: the head-counting kernel on the
+  // entry-cut blocks, one step narrower when those leave fewer than 16
+  // workgroups per CU
+  if (K == 1 && (!useN || p->aliased) && !p->partT[0].rowsInKey && p->nnzT > 0) {
+    const Partition *vp = vec_partition<T>(p, 1, 0);
+    if (vp && vp->nblocks < 16 * p->cus && vp + 1 < p->partT + p->nparts && vp + 1 < p->partT + kPackParts) ++vp;
+    if (vp) {
+      const ListView G = list_view(p, false, vp, false);
+      const T *v = static_cast<const T *>(d_v);
+      T *gv = static_cast<T *>(d_gv);
+#define OGS_GV_LAUNCH(OPV)                                                                                     \
+  hipLaunchKernelGGL((k_gather_vec<T, OPV>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, G.pk, \
+                     G.base, G.stride, v, gv);
+      switch (op) {
+        case OGS_ADD: OGS_GV_LAUNCH(OGS_ADD) break;
+        case OGS_MUL:
+          if constexpr (kOffered<T, OGS_MUL>) { OGS_GV_LAUNCH(OGS_MUL) } else { return OGS_E_NOT_SUPPORTED; }
+          break;
+        case OGS_MAX: OGS_GV_LAUNCH(OGS_MAX) break;
+        case OGS_MIN: OGS_GV_LAUNCH(OGS_MIN) break;
+        default: return OGS_E_ARG;
+      }
+#undef OGS_GV_LAUNCH
+      return (int)hipGetLastError();
+    }
+  }
   const bool partN = useN && !p->aliased;
   const Partition *parts = partN ? p->partN : p->partT;
   int kc = 0;
