@@ -1,21 +1,29 @@
 /*
  * ogs_hip.h -- C-ABI of the MI355X (gfx950) rank-local gather-scatter.
  *
- * Two layers, both in libogs_hip.so:
+ * Three layers, all in libogs_hip.so:
  *
  *  1. Operator layer (ogs_hip_*): the device half of ogsOperator_t.  A plan
  *     wraps one caller-owned CSR pair (N and T lists) on the device and owns
- *     only the block partitions it derives from them.  It replaces the
- *     OCCA kernel tables and device overloads of the reference:
+ *     only what it derives from them (row partitions, column-sorted packs of
+ *     the lists).  It replaces the OCCA kernel tables and device overloads
+ *     of the reference:
  *       ogsOperator_t::Gather  (deviceMemory)  libs/ogs/ogsOperator.cpp:159-187
  *       ogsOperator_t::Scatter (deviceMemory)  libs/ogs/ogsOperator.cpp:266-293
  *       ogsOperator_t::GatherScatter (device)  libs/ogs/ogsOperator.cpp:393-432
  *       OKL gatherScatter/gather/scatter/extract  libs/ogs/okl/ogsKernels.okl:32-177
  *       InitializeKernels (JIT tables)          libs/ogs/ogsUtils.cpp:46-123
  *
+ *     With it, the batch entry points (ogs_hip_batch_*): many (plan, K,
+ *     buffer) GatherScatters in one HIP graph launch, no counterpart in the
+ *     reference.
+ *
  *  2. Handle layer (ogs_*): the ogs_t object of include/ogs.hpp:216-344 at
  *     one rank -- Setup from a global numbering, then GatherScatter / Gather
  *     / Scatter on device buffers.
+ *
+ *  3. Halo layer (halo_*): the halo_t object of include/ogs.hpp:347-397 at
+ *     one rank.
  *
  * Conventions (all entry points):
  *  - Device buffers are plain device pointers.  `stream` is a hipStream_t
