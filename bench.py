@@ -381,8 +381,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        log("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
-            "rank-local gather-scatter has no multi-GPU path here (see DESIGN.md, 'Sharding').")
+        log("bench.py measures one GPU only: the rank-local gather-scatter runs on one MI355X and "
+            "has no multi-GPU path (several GPUs would each run it on their own sub-box).")
         return 2
 
     import torch

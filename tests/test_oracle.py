@@ -16,7 +16,7 @@ import pytest
 
 from oracle import oracle as O
 
-SURVEY_COUNTS = {  # (nx, p): (N, Ngather, {len: rows}, NrowBlocksT)
+CONFIG_COUNTS = {  # (nx, p): (N, Ngather, {len: rows}, NrowBlocksT)
     (10, 3): (64000, 29791, {1: 10648, 2: 13068, 4: 5346, 8: 729}, 126),
     (24, 3): (884736, 389017, {1: 125000, 2: 172500, 4: 79350, 8: 12167}, 1729),
 }
@@ -116,9 +116,9 @@ def test_setup_brute_force(box, kind, unique, flag, rng):
         assert np.array_equal(o.ids, want_ids)
 
 
-@pytest.mark.parametrize("nx,p", list(SURVEY_COUNTS))
+@pytest.mark.parametrize("nx,p", list(CONFIG_COUNTS))
 def test_setup_counts_match_reference_configs(nx, p):
-    N, Ng, hist, nblk = SURVEY_COUNTS[(nx, p)]
+    N, Ng, hist, nblk = CONFIG_COUNTS[(nx, p)]
     ids = O.make_ids(nx, nx, nx, p)
     o = O.OracleOgs(ids, O.UNSIGNED)
     assert o.N == N and o.NgatherGlobal == Ng == (p * nx + 1) ** 3
