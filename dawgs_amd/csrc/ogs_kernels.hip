@@ -775,13 +775,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_gs_rest(const BatchDev *__rest
 //
 // Shape per (T, KT): VW fields per work item, SL slots per lane, sized so
 // that the values in flight stay within ~24 VGPRs; a block holds at most CAP
-// entries (slots, LDS and the pack's 2048 positions all bound it).  Three
-// 8-byte fields go one per item (measured 3-8% faster than a 24-byte item,
-// which a 16-byte load cannot cover aligned); three 4- or 2-byte fields go
-// in one item (one field per item measured ~1.8x slower there).
+// entries (slots, LDS and the pack's 2048 positions all bound it).  8-byte
+// fields go two per item (one aligned 16-byte access; measured 3-7% faster
+// than four per item at K = 4..16) and, at K = 3, one per item (3-8% faster
+// than a 24-byte item); 4- and 2-byte fields go up to four per item (one
+// field per item measured ~1.8x slower at K = 3).
 template <typename T, int KT>
 struct VecShape {
-  static constexpr int VW = (KT == 3 && sizeof(T) == 8) ? 1 : vec_width(KT);
+  static constexpr int VW = sizeof(T) == 8 ? (KT == 3 ? 1 : KT >= 2 ? 2 : 1) : vec_width(KT);
   static constexpr int C = KT / VW;
   static constexpr int WORDS = (int)((sizeof(T) + 3) / 4);
   static constexpr int SL = VW * WORDS <= 3 ? 8 : VW * WORDS <= 6 ? 4 : 2;
