@@ -123,12 +123,15 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
                     const void *d_q, void *d_gq, void *stream);
 
 /* Plan facts, for reporting: block count and width of the K=1 partition
- * used by GatherScatter, the longest row, nnz of both lists, and how many
- * K=1 blocks of the T list read their columns in the plan's column-sorted
- * order (packedK1) or in CSR order (plainK1). */
+ * used by GatherScatter, the longest row, nnz of both lists, how many K=1
+ * blocks of the T list read their columns in the plan's column-sorted order
+ * (packedK1) or in CSR order (plainK1), and the layer distance (in blocks)
+ * that the K=1 GatherScatter's block order is tiled for (tileD; 0 = block id
+ * order).  Plan creation picks the tiled order only where it measures
+ * faster, on scratch values. */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
-  int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols;
+  int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
 
