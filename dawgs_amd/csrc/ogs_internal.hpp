@@ -41,9 +41,12 @@ struct Partition {
   int32_t maxT = 0;     // largest T-list nonzero count of a block
   int32_t maxN = 0;     // largest N-list nonzero count of a block
   int32_t maxRows = 0;  // largest row count of a block
-  int *order = nullptr;  // block run order of the K=1 GatherScatter (null: block id order)
+  int *order = nullptr;  // tiled block run order (null: block id order)
   int32_t tileD = 0;     // layer distance (blocks) the order is tiled for, 0: none
+  int32_t tiled = 0;     // kernels that run in `order`: kTiledGS | kTiledGather | kTiledScatter
 };
+
+constexpr int kTiledGS = 1, kTiledGather = 2, kTiledScatter = 4;  // Partition::tiled bits
 
 // A list's entries in the blocks of one partition, each block's entries
 // sorted by column and packed; base[b] holds the block's segment bases, with

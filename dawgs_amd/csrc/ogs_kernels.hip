@@ -1044,7 +1044,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
                                                        const int32_t *__restrict__ ids,
                                                        const uint32_t *__restrict__ pk,
                                                        const int4 *__restrict__ base, int stride,
-                                                       const T *__restrict__ q, T *__restrict__ gq) {
+                                                       const T *__restrict__ q, T *__restrict__ gq,
+                                                       const int *__restrict__ order) {
   using V = VecShape<T, 1>;
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
@@ -1061,7 +1062,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
   VecStage<SL> s;
   VecMeta m;
   T x[SL][1];
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
   vec_meta(src, b, m);
   vec_index<T, 1>(s, src, b, m);
   vec_gather<T, 1>(s, x);
@@ -1134,7 +1136,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__
                                                         const int32_t *__restrict__ ids,
                                                         const uint32_t *__restrict__ pk,
                                                         const int4 *__restrict__ base, int stride,
-                                                        const T *__restrict__ gq, T *__restrict__ q) {
+                                                        const T *__restrict__ gq, T *__restrict__ q,
+                                                        const int *__restrict__ order) {
   constexpr int SL = kSlots;  // kCap entries and at most kCap rows per block
   constexpr int NW = kCap / 32;
   static_assert(NW == 64, "one wave scans the head-mask words");
@@ -1146,7 +1149,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__
   const VecSrc src = {part, ids, pk, base, q, 0, stride};
   VecStage<SL> s;
   VecMeta m;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
   vec_meta(src, b, m);
   vec_index<T, 1>(s, src, b, m);
   const int n = s.n;
@@ -1715,7 +1719,7 @@ static const Partition *vec_partition(const Plan *p, int K, int narrow) {
 template <typename T, int OP, int KT>
 static void launch_vec_k(const Partition *pt, const ListView &G, int gN, T *v, hipStream_t st) {
   hipLaunchKernelGGL((k_gs_sym_vec<T, OP, KT>), dim3(pt->nblocks), dim3(kBlock), 0, st, pt->d, G.starts, G.ids, G.pk,
-                     G.base, G.stride, gN, v, pt->order);
+                     G.base, G.stride, gN, v, (pt->tiled & kTiledGS) ? pt->order : nullptr);
 }
 
 template <typename T, int OP>
@@ -1828,7 +1832,7 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
       T *gv = static_cast<T *>(d_gv);
 #define OGS_GV_LAUNCH(OPV)                                                                                     \
   hipLaunchKernelGGL((k_gather_vec<T, OPV>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, G.pk, \
-                     G.base, G.stride, v, gv);
+                     G.base, G.stride, v, gv, (vp->tiled & kTiledGather) ? vp->order : nullptr);
       switch (op) {
         case OGS_ADD: OGS_GV_LAUNCH(OGS_ADD) break;
         case OGS_MUL:
@@ -1887,7 +1891,8 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
     if (vp->nblocks < 16 * p->cus && p->nparts > 1) vp = &p->partT[1];
     const ListView S = list_view(p, false, vp, false);
     hipLaunchKernelGGL((k_scatter_vec<T>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
-                       S.base, S.stride, static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+                       S.base, S.stride, static_cast<const T *>(d_gv), static_cast<T *>(d_v),
+                       (vp->tiled & kTiledScatter) ? vp->order : nullptr);
     return (int)hipGetLastError();
   }
   const Partition *parts = partN ? p->partN : p->partS;
@@ -1969,16 +1974,10 @@ static std::vector<int> tiled_order(int n, int D) {
   return order;
 }
 
-// Time `reps` fp64 Add GatherScatter launches of partition 0 over scratch values.
-static int time_k1(const Plan *p, const int *order, double *vals, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
-                   float *us) {
-  const Partition &pt = p->partT[0];
-  const Pack &pk = p->packT[0];
+// Mean time of 8 launches (after one untimed) of `launch`.
+template <class F>
+static int time_launches(F launch, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float *us) {
   constexpr int reps = 8;
-  auto launch = [&]() {
-    hipLaunchKernelGGL((k_gs_sym_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT,
-                       pk.pk, pk.base, pk.stride, 0, vals, order);
-  };
   launch();
   int rc = (int)hipEventRecord(e0, st);
   for (int i = 0; i < reps && !rc; ++i) launch();
@@ -1990,14 +1989,30 @@ static int time_k1(const Plan *p, const int *order, double *vals, hipStream_t st
   return rc;
 }
 
-// The K=1 GatherScatter's block order: the tiled order for the list's layer
-// distance when one stands out and the tiled order measures at least 5%
-// faster on scratch values (best of three passes each, the candidates'
-// order alternating, as the first launches after a setup run slow); else
-// block id order.
-static int tile_schedule(Plan *p, hipStream_t st) {
-  Partition &pt = p->partT[0];
-  if (p->nparts == 0 || pt.nblocks < 16 * p->cus || p->Ncols < 2 || p->nnzT == 0 || pt.maxT > kCap) return 0;
+// Whether `order` makes `launch(order)` at least 5% faster than block id
+// order: best of three passes each, the candidates' order alternating (the
+// first launches after a setup run slow).
+template <class F>
+static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool *pays) {
+  float best[2] = {0.f, 0.f};
+  int rc = 0;
+  for (int pass = 0; pass < 3 && !rc; ++pass) {
+    for (int i = 0; i < 2 && !rc; ++i) {
+      const int c = (pass & 1) ? 1 - i : i;
+      float us = 0.f;
+      rc = time_launches([&]() { launch(c ? order : nullptr); }, st, e0, e1, &us);
+      if (!rc && (best[c] == 0.f || us < best[c])) best[c] = us;
+    }
+  }
+  *pays = !rc && best[1] < 0.95f * best[0];
+  return rc;
+}
+
+// The most common block distance D >= kTileDMin between the blocks of
+// adjacent columns of the T list over partition `pt`, or 0 when none covers
+// 0.5% of the columns.
+static int layer_distance(const Plan *p, const Partition &pt, hipStream_t st, int *D) {
+  *D = 0;
   int32_t *blk = nullptr;
   uint32_t *hist = nullptr;
   int rc = (int)hipMalloc(&blk, sizeof(int32_t) * (size_t)p->Ncols);
@@ -2015,37 +2030,85 @@ static int tile_schedule(Plan *p, hipStream_t st) {
   if (blk) (void)hipFree(blk);
   if (hist) (void)hipFree(hist);
   if (rc) return rc;
-  int D = 0;
+  int best = 0;
   for (int d = kTileDMin; d < kTileDMax; ++d)
-    if (!D || h[d] > h[D]) D = d;
-  if (!D || (double)h[D] < 0.005 * p->Ncols) return 0;  // no layer distance stands out
-  const std::vector<int> order = tiled_order(pt.nblocks, D);
-  int *d_order = nullptr;
-  double *vals = nullptr;
+    if (!best || h[d] > h[best]) best = d;
+  if (best && (double)h[best] >= 0.005 * p->Ncols) *D = best;
+  return 0;
+}
+
+// Tiled block orders of the K = 1 kernels of a T list without empty rows:
+// for the GatherScatter and Gather over partT[0] and the Scatter over
+// partS[0], each partition's layer distance, its tiled order, and per kernel
+// whether that order measures faster on scratch values (fp64).  Plans whose
+// widest partition has fewer than 16 blocks per CU keep block id order.
+static int tile_schedule(Plan *p, hipStream_t st) {
+  Partition &pt = p->partT[0];
+  Partition &ps = p->partS[0];
+  if (p->nparts == 0 || pt.nblocks < 16 * p->cus || p->Ncols < 2 || p->nnzT == 0 || pt.maxT > kCap ||
+      ps.maxT > kCap || ps.d == pt.d)
+    return 0;
+  int DT = 0, DS = 0;
+  int rc = layer_distance(p, pt, st, &DT);
+  if (!rc) rc = layer_distance(p, ps, st, &DS);
+  if (rc || (!DT && !DS)) return rc;
+  double *vals = nullptr, *rows = nullptr;
+  int *oT = nullptr, *oS = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  rc = (int)hipMalloc(&d_order, sizeof(int) * order.size());
-  if (!rc) rc = (int)hipMemcpyAsync(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st);
-  if (!rc) rc = (int)hipMalloc(&vals, sizeof(double) * (size_t)p->Ncols);
+  rc = (int)hipMalloc(&vals, sizeof(double) * (size_t)p->Ncols);
+  if (!rc) rc = (int)hipMalloc(&rows, sizeof(double) * (size_t)p->NrowsT);
   if (!rc) rc = (int)hipMemsetAsync(vals, 0, sizeof(double) * (size_t)p->Ncols, st);
+  if (!rc) rc = (int)hipMemsetAsync(rows, 0, sizeof(double) * (size_t)p->NrowsT, st);
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
-  float best[2] = {0.f, 0.f};
-  for (int pass = 0; pass < 3 && !rc; ++pass) {
-    for (int i = 0; i < 2 && !rc; ++i) {
-      const int c = (pass & 1) ? 1 - i : i;
-      float us = 0.f;
-      rc = time_k1(p, c ? d_order : nullptr, vals, st, e0, e1, &us);
-      if (!rc && (best[c] == 0.f || us < best[c])) best[c] = us;
-    }
+  auto upload = [&](const Partition &x, int D, int **out) {
+    const std::vector<int> order = tiled_order(x.nblocks, D);
+    int r = (int)hipMalloc(out, sizeof(int) * order.size());
+    if (!r) r = (int)hipMemcpyAsync(*out, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st);
+    if (!r) r = (int)hipStreamSynchronize(st);
+    return r;
+  };
+  bool gs = false, ga = false, sc = false;
+  if (!rc && DT) {
+    rc = upload(pt, DT, &oT);
+    const Pack &k = p->packT[0];
+    if (!rc)
+      rc = order_pays([&](const int *o) {
+        hipLaunchKernelGGL((k_gs_sym_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
+                           p->ciT, k.pk, k.base, k.stride, 0, vals, o);
+      }, oT, st, e0, e1, &gs);
+    if (!rc)
+      rc = order_pays([&](const int *o) {
+        hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
+                           p->ciT, k.pk, k.base, k.stride, vals, rows, o);
+      }, oT, st, e0, e1, &ga);
+  }
+  if (!rc && DS) {
+    rc = upload(ps, DS, &oS);
+    const Pack &k = p->packS;
+    if (!rc)
+      rc = order_pays([&](const int *o) {
+        hipLaunchKernelGGL((k_scatter_vec<double>), dim3(ps.nblocks), dim3(kBlock), 0, st, ps.d, p->rsT, p->ciT,
+                           k.pk, k.base, k.stride, rows, vals, o);
+      }, oS, st, e0, e1, &sc);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (vals) (void)hipFree(vals);
-  if (!rc && best[1] < 0.95f * best[0]) {
-    pt.order = d_order;
-    pt.tileD = D;
-  } else if (d_order) {
-    (void)hipFree(d_order);
+  if (rows) (void)hipFree(rows);
+  if (!rc && (gs || ga)) {
+    pt.order = oT;
+    pt.tileD = DT;
+    pt.tiled = (gs ? kTiledGS : 0) | (ga ? kTiledGather : 0);
+  } else if (oT) {
+    (void)hipFree(oT);
+  }
+  if (!rc && sc) {
+    ps.order = oS;
+    ps.tileD = DS;
+    ps.tiled = kTiledScatter;
+  } else if (oS) {
+    (void)hipFree(oS);
   }
   return rc;
 }
@@ -2115,6 +2178,7 @@ static void free_plan(Plan *p) {
   if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
   if (p->partT[0].order) (void)hipFree(p->partT[0].order);
+  if (p->partS[0].order && p->partS[0].order != p->partT[0].order) (void)hipFree(p->partS[0].order);
   for (int j = 0; j < p->nparts; ++j) {
     if (p->partT[j].d) (void)hipFree(p->partT[j].d);
     if (p->partN[j].d && p->partN[j].d != p->partT[j].d) (void)hipFree(p->partN[j].d);

@@ -126,9 +126,9 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * used by GatherScatter, the longest row, nnz of both lists, how many K=1
  * blocks of the T list read their columns in the plan's column-sorted order
  * (packedK1) or in CSR order (plainK1), and the layer distance (in blocks)
- * that the K=1 GatherScatter's block order is tiled for (tileD; 0 = block id
- * order).  Plan creation picks the tiled order only where it measures
- * faster, on scratch values. */
+ * that the K=1 GatherScatter / Gather block order is tiled for (tileD; 0 =
+ * block id order).  Plan creation picks a tiled order, per kernel, only where
+ * it measures faster on scratch values. */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
