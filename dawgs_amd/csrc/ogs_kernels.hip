@@ -2015,8 +2015,13 @@ static int layer_distance(const Plan *p, const Partition &pt, hipStream_t st, in
   *D = 0;
   int32_t *blk = nullptr;
   uint32_t *hist = nullptr;
-  int rc = (int)hipMalloc(&blk, sizeof(int32_t) * (size_t)p->Ncols);
-  if (!rc) rc = (int)hipMalloc(&hist, sizeof(uint32_t) * kTileDMax);
+  if (hipMalloc(&blk, sizeof(int32_t) * (size_t)p->Ncols) != hipSuccess ||
+      hipMalloc(&hist, sizeof(uint32_t) * kTileDMax) != hipSuccess) {
+    (void)hipGetLastError();  // no room for the histogram: keep block id order
+    if (blk) (void)hipFree(blk);
+    return 0;
+  }
+  int rc = 0;
   if (!rc) rc = (int)hipMemsetAsync(blk, 0xff, sizeof(int32_t) * (size_t)p->Ncols, st);
   if (!rc) rc = (int)hipMemsetAsync(hist, 0, sizeof(uint32_t) * kTileDMax, st);
   if (!rc) {
@@ -2055,9 +2060,13 @@ static int tile_schedule(Plan *p, hipStream_t st) {
   double *vals = nullptr, *rows = nullptr;
   int *oT = nullptr, *oS = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  rc = (int)hipMalloc(&vals, sizeof(double) * (size_t)p->Ncols);
-  if (!rc) rc = (int)hipMalloc(&rows, sizeof(double) * (size_t)p->NrowsT);
-  if (!rc) rc = (int)hipMemsetAsync(vals, 0, sizeof(double) * (size_t)p->Ncols, st);
+  if (hipMalloc(&vals, sizeof(double) * (size_t)p->Ncols) != hipSuccess ||
+      hipMalloc(&rows, sizeof(double) * (size_t)p->NrowsT) != hipSuccess) {
+    (void)hipGetLastError();  // no room for scratch values: keep block id order
+    if (vals) (void)hipFree(vals);
+    return 0;
+  }
+  rc = (int)hipMemsetAsync(vals, 0, sizeof(double) * (size_t)p->Ncols, st);
   if (!rc) rc = (int)hipMemsetAsync(rows, 0, sizeof(double) * (size_t)p->NrowsT, st);
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
