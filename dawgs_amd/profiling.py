@@ -49,14 +49,20 @@ def trace_tail_mean(d: str, kernel: str, last: int) -> dict:
             "all_mean_ns": sum(durs) / max(len(durs), 1)}
 
 
-def counter_means(d: str, counter: str):
-    """Mean counter value (KiB as rocprofv3 reports it) per kernel name."""
+def counter_means(d: str, counter: str, last: int = 0):
+    """Mean counter value (KiB as rocprofv3 reports it) per kernel name, over
+    each kernel's last `last` dispatches (all when 0)."""
     acc = {}
     for r in _rows(d, "counter_collection.csv"):
         if r["Counter_Name"] != counter:
             continue
-        acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+        acc.setdefault(r["Kernel_Name"], []).append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    out = {}
+    for k, v in acc.items():
+        v.sort()
+        vals = [x for _, x in (v[-last:] if last else v)]
+        out[k] = sum(vals) / len(vals)
+    return out
 
 
 def _pick(means: dict, pattern: str, exclude: str = "rocclr") -> float:
@@ -66,14 +72,16 @@ def _pick(means: dict, pattern: str, exclude: str = "rocclr") -> float:
     return hits[0]
 
 
-def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_kernel: str, copy_bytes: int) -> dict:
-    f = counter_means(fetch_dir, "FETCH_SIZE")
-    w = counter_means(write_dir, "WRITE_SIZE")
+def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_kernel: str, copy_bytes: int, last: int = 0) -> dict:
+    """HBM bytes per launch of `kernel`, over its last `last` dispatches (all
+    when 0: plan creation may launch a kernel too, to time block orders)."""
+    f, w = counter_means(fetch_dir, "FETCH_SIZE"), counter_means(write_dir, "WRITE_SIZE")
+    fk, wk = counter_means(fetch_dir, "FETCH_SIZE", last), counter_means(write_dir, "WRITE_SIZE", last)
     half = copy_bytes / 2  # a copy reads half and writes half of the bytes it moves
     f_scale = half / (_pick(f, copy_kernel) * 1024)
     w_scale = half / (_pick(w, copy_kernel) * 1024)
-    rd = _pick(f, kernel) * 1024 * f_scale
-    wr = _pick(w, kernel) * 1024 * w_scale
+    rd = _pick(fk, kernel) * 1024 * f_scale
+    wr = _pick(wk, kernel) * 1024 * w_scale
     return {"kernel": kernel, "fetch_bytes": round(rd), "write_bytes": round(wr),
             "hbm_bytes_per_launch": round(rd + wr), "fetch_scale": round(f_scale, 4),
             "write_scale": round(w_scale, 4), "copy_kernel": copy_kernel, "copy_bytes": copy_bytes}
@@ -94,6 +102,7 @@ def main(argv=None) -> int:
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--last", type=int, default=0, help="only each kernel's last N dispatches")
     t.add_argument("--json")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
@@ -102,7 +111,7 @@ def main(argv=None) -> int:
     elif a.cmd == "tail":
         print(json.dumps(trace_tail_mean(a.dir, a.kernel, a.last), indent=1))
     else:
-        r = traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_kernel, a.copy_bytes)
+        r = traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_kernel, a.copy_bytes, a.last)
         print(json.dumps(r, indent=1))
         if a.json:
             with open(a.json, "w") as fh:
