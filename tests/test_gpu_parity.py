@@ -315,6 +315,16 @@ def test_tiled_block_order_192cubed_p1(torch, D):
     expect = np.empty(N)
     expect[c["colIdsT"]] = np.repeat(np.diff(c["rowStartsT"]).astype(np.float64), np.diff(c["rowStartsT"]))
     assert same(to_host(torch, ones), expect)
+    del ones, expect
+    # plan-level Gather Trans and Scatter NoTrans (each with its own block-order choice)
+    wg = ref.gather(v, 1, O.ADD, O.TRANS)
+    gq = torch.zeros(len(wg), dtype=torch.float64, device="cuda")
+    g.plan.Gather(gq, to_dev(torch, v), 1, O.ADD, O.TRANS)
+    assert same(to_host(torch, gq), wg)
+    base = rng.uniform(-1, 1, N)
+    d = to_dev(torch, base)
+    g.plan.Scatter(d, gq, 1, O.NOTRANS)
+    assert same(to_host(torch, d), ref.scatter(base.copy(), wg, 1, O.NOTRANS))
 
 
 # ------------------------------------------------------------------ edge cases
