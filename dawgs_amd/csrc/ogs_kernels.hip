@@ -2606,7 +2606,7 @@ static void free_batch(ogs_hip_batch_s *b) {
   delete b;
 }
 
-// Times a configuration on scratch buffers: 1 warm-up, 3 timed launches.
+// Times a configuration on scratch buffers: 1 warm-up, kTuneLaunches timed launches.
 constexpr int kTuneLaunches = 5;  // timed launches per candidate and pass
 
 static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
