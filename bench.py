@@ -19,7 +19,7 @@ oracle) and `batched` (48 meshes in one graph launch against one launch per
 mesh and field); --no-extras skips them.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 100] [--nx 48] [--p 7] [--no-cpu] [--copy-probe]
-    python bench.py --sweep [DEGREES] | --batched | --precision
+    python bench.py --sweep [DEGREES] | --batched | --precision | --readme
 """
 from __future__ import annotations
 
@@ -148,6 +148,41 @@ def run_sweep(args) -> int:
                      "scatter_DOFSps": r["NgatherGlobal"] / r["scatter"]["s"], "scatter_GBps": r["scatter"]["GBps"],
                      "host_gs_DOFSps": (r["NgatherGlobal"] / r["host_s"]["gs"]) if "host_s" in r else None}
                     for r in rows]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+# The reference README's sample run (-sw: one rank, 34^3 elements, p = 1):
+# DOFS/s = NgatherGlobal / time per call, on hardware it does not name
+# (README.md:78-83).
+README_SAMPLE = {"nx": 34, "p": 1, "global_dofs": 42875, "local_dofs": 314432,
+                 "DOFSps": {"device_gs": 2.21005e9, "device_gather": 2.9569e9, "device_scatter": 2.82072e9,
+                            "host_gs": 2.38924e7, "host_gather": 3.78253e7, "host_scatter": 3.28872e7}}
+
+
+def run_readme(args) -> int:
+    """The README's sample configuration timed as PerformanceTest times it
+    (10 warm-ups, then the mean of 20 calls; dawgsMain.cpp:146-311), Device
+    and (unless --no-cpu) Host lines in its format, and one JSON line with
+    the README's published figures beside this build's."""
+    from dawgs_amd.perf import performance_test
+    from oracle import oracle as O
+    r = README_SAMPLE
+    res = performance_test(r["nx"], r["nx"], r["nx"], r["p"], warmup=10, steps=20, log=log)
+    assert res["NgatherGlobal"] == r["global_dofs"] and res["N"] == r["local_dofs"]
+    ng = res["NgatherGlobal"]
+    got = {"device_gs": ng / res["gs"]["s"], "device_gather": ng / res["gather"]["s"],
+           "device_scatter": ng / res["scatter"]["s"]}
+    if not args.no_cpu:
+        h = host_lines(res, log)
+        got.update(host_gs=ng / h["gs"], host_gather=ng / h["gather"], host_scatter=ng / h["scatter"])
+    out = {"metric": "README sample run (34^3 elements, p=1, one rank): DOFS/s per operation", "unit": "DOFS/s",
+           "n_gpus": 1, "steps": 20, "warmup": 10, "dtype": "f64", "data": "synthetic",
+           "config": {"nx": r["nx"], "p": r["p"], "local_dofs": r["local_dofs"], "global_dofs": r["global_dofs"]},
+           "this_build": got, "readme_published": r["DOFSps"],
+           "ratio": {k: got[k] / r["DOFSps"][k] for k in got},
+           "host_threads": O.num_threads() if not args.no_cpu else None,
+           "host_cpu": host_cpu() if not args.no_cpu else None}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -374,6 +409,9 @@ def main() -> int:
                     help="precision form: FP32-compensated and BF16 storage, K=3 fields on the 48^3 p=7 mesh, "
                          "values log-uniform over [1e-3,1e3] with random sign; errors against the fp64 oracle "
                          "and DOFS/s against the plain fp64 kernel; one JSON line on stdout")
+    ap.add_argument("--readme", action="store_true",
+                    help="the reference README's sample run (34^3 elements, p=1): Device/Host lines in its "
+                         "format and one JSON line with its published DOFS/s beside this build's")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, time a plain device copy of the same byte count "
                          "(the practical streaming ceiling, and a known-size kernel for scaling "
@@ -400,6 +438,8 @@ def main() -> int:
         return run_batched(args)
     if args.precision:
         return run_precision(args)
+    if args.readme:
+        return run_readme(args)
     nx, p = args.nx, args.p
 
     t0 = time.perf_counter()

@@ -27,17 +27,27 @@ Auto, Pairwise, CrystalRouter, AllToAll = 0, 1, 2, 3
 Unsigned, Signed, Halo = 0, 1, 2
 
 
+_TYPES = None  # torch dtype -> type code, built on first use (torch imported lazily)
+
+
+def _types() -> dict:
+    global _TYPES
+    if _TYPES is None:
+        import torch
+        _TYPES = {torch.float32: Float, torch.float64: Double, torch.int32: Int32, torch.int64: Int64,
+                  torch.bfloat16: BF16}
+    return _TYPES
+
+
 def type_of(t, type=None) -> int:
     """Type code of a buffer: the one its dtype names, or `type` when given
     (FloatC selects the compensated Add on a float32 buffer)."""
-    import torch
-    m = {torch.float32: Float, torch.float64: Double, torch.int32: Int32, torch.int64: Int64,
-         torch.bfloat16: BF16}
-    if t.dtype not in m:
+    code = _types().get(t.dtype)
+    if code is None:
         raise TypeError(f"unsupported dtype {t.dtype}")
-    if type is None:
-        return m[t.dtype]
-    if not (type == m[t.dtype] or (type == FloatC and t.dtype == torch.float32)):
+    if type is None or type == code:
+        return code
+    if not (type == FloatC and code == Float):
         raise TypeError(f"type {type} does not store {t.dtype}")
     return type
 
@@ -52,11 +62,23 @@ def _dev_ptr(t) -> int:
     return t.data_ptr()
 
 
+_RAW_STREAM = None  # (current-device getter, raw current-stream getter), found on first use
+
+
 def _stream(stream) -> int:
+    """The given stream, or torch's current stream on the current device."""
+    global _RAW_STREAM
     if stream is not None:
         return stream
-    import torch
-    return torch.cuda.current_stream().cuda_stream
+    if _RAW_STREAM is None:
+        import torch
+        C = torch._C
+        if hasattr(C, "_cuda_getCurrentRawStream") and hasattr(C, "_cuda_getDevice"):
+            _RAW_STREAM = (C._cuda_getDevice, C._cuda_getCurrentRawStream)
+        else:  # public API (slower: builds a Stream object per call)
+            _RAW_STREAM = (torch.cuda.current_device, lambda d: torch.cuda.current_stream(d).cuda_stream)
+    dev, raw = _RAW_STREAM
+    return raw(dev())
 
 
 class Plan:
