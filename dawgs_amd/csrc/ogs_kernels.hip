@@ -2247,6 +2247,13 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     return rc;
   }
   if (t0 != 0 || n0 != 0 || tN < 0 || nN < 0) { delete p; return OGS_E_CSR; }
+  // rows [NrowsN, NrowsT) of the N list are empty (the whole-partition
+  // kernels of an aliased operator rely on it)
+  if (NrowsN < NrowsT) {
+    int32_t nR = 0;
+    if ((rc = read_int(d_rowStartsN + NrowsN, st, &nR))) { delete p; return rc; }
+    if (nR != nN) { delete p; return OGS_E_CSR; }
+  }
   p->nnzT = tN;
   p->nnzN = nN;
   if ((p->nnzT > 0 && !d_colIdsT) || (p->nnzN > 0 && !d_colIdsN)) { delete p; return OGS_E_ARG; }

@@ -392,6 +392,27 @@ def test_row_limits_and_errors(torch, D):
     assert e.value.code == -3
 
 
+def test_plan_rejects_entries_in_n_rows_beyond_nrowsn(torch, D):
+    """Rows [NrowsN, NrowsT) of the N list must be empty (ogs_hip_plan_create's
+    contract): a list with entries there is refused, for an aliased operator
+    and for distinct lists."""
+    from dawgs_amd._lib import OgsError
+    E_CSR = -7
+    rs = np.array([0, 2, 3, 5], np.int32)  # 3 rows, the last one non-empty
+    ci = np.array([0, 4, 1, 2, 3], np.int32)
+    d_rs, d_ci = to_dev(torch, rs), to_dev(torch, ci)
+    with pytest.raises(OgsError) as e:
+        D.Plan(5, 2, 3, d_rs, d_ci, d_rs, d_ci)
+    assert e.value.code == E_CSR
+    rsN = np.array([0, 1, 2, 3], np.int32)
+    ciN = np.array([0, 1, 3], np.int32)
+    with pytest.raises(OgsError) as e:
+        D.Plan(5, 2, 3, to_dev(torch, rsN), to_dev(torch, ciN), d_rs, d_ci)
+    assert e.value.code == E_CSR
+    ok = D.Plan(5, 3, 3, d_rs, d_ci, d_rs, d_ci)  # all rows in N: accepted
+    assert ok.info()["NrowsN"] == 3
+
+
 def test_hand_built_operator_long_rows(torch, D):
     """Plan layer on a hand-built Signed operator (the halo pack/unpack shape):
     rows of 1..1500 members, distinct N and T lists, K = 1 and 3."""
