@@ -92,10 +92,11 @@ typedef struct ogs_hip_plan_s *ogs_hip_plan_t;
 /* Wrap a CSR pair already on the device.  Rows [0,NrowsT) are described by
  * d_rowStartsT/d_colIdsT (all members) and d_rowStartsN/d_colIdsN (positive
  * members only); both row-start arrays have NrowsT+1 entries and rows
- * [NrowsN, NrowsT) of the N list are empty.  Pass the same pointers for N and
- * T for an Unsigned operator (ogsSetup.cpp:719,728).  Synchronises `stream`.
- * Replaces ogsOperator_t::setupRowBlocks' device upload (ogsOperator.cpp:510-511)
- * with the plan's own partitions. */
+ * [NrowsN, NrowsT) of the N list are empty (else OGS_E_CSR, as for
+ * decreasing row starts or a column outside [0, Ncols)).  Pass the same
+ * pointers for N and T for an Unsigned operator (ogsSetup.cpp:719,728).
+ * Synchronises `stream`.  Replaces ogsOperator_t::setupRowBlocks' device
+ * upload (ogsOperator.cpp:510-511) with the plan's own partitions. */
 int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
                         const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
                         const int32_t *d_rowStartsT, const int32_t *d_colIdsT, void *stream);
