@@ -1702,6 +1702,19 @@ static int vec_cap_entries(int K) {
   }
 }
 
+// Workgroups per CU a launch needs to fill the device: on small meshes each
+// step to narrower blocks up to this count measured faster (fewer entries per
+// block, shorter dependent chains per CU), past it wider blocks were.
+constexpr int kFillPerCu = 4;
+
+// From T partition `vp`, the narrower partitions with packs, step by step,
+// until the launch has kFillPerCu workgroups per CU.
+static const Partition *fill_device(const Plan *p, const Partition *vp) {
+  const Partition *end = p->partT + (p->nparts < kPackParts ? p->nparts : kPackParts);
+  while (vp->nblocks < kFillPerCu * p->cus && vp + 1 < end) ++vp;
+  return vp;
+}
+
 // The widest T partition whose blocks hold at most the shape's entries, made
 // `narrow` steps narrower when there is one.
 template <typename T>
@@ -1786,12 +1799,12 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   const ListView G = list_view(p, gListN, pt, false), S = list_view(p, sListN, pt, false);
   // one list (rows [0, NrowsT)) and a field count with a compile-time
   // shape: the entry-major kernel on the widest partition whose blocks fit
-  // that shape (for K = 1 partition 0 always fits), one step narrower when
-  // that leaves fewer than 16 workgroups per CU to spread.
+  // that shape (for K = 1 partition 0 always fits), narrowed to fill the
+  // device on small lists.
   if (sym && p->nnzT == 0) return OGS_SUCCESS;  // no entries: nothing to write
   if (sym && vec_k(K)) {
     const Partition *vp = vec_partition<T>(p, K, 0);
-    if (vp && vp->nblocks < 16 * p->cus && vp + 1 < p->partT + p->nparts && vp + 1 < p->partT + kPackParts) ++vp;
+    if (vp) vp = fill_device(p, vp);
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
 #define OGS_GS_LAUNCH(OPV)                                                                                      \
@@ -1821,11 +1834,10 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
   // K = 1 from a T list without empty rows: the head-counting kernel on the
-  // entry-cut blocks, one step narrower when those leave fewer than 16
-  // workgroups per CU
+  // entry-cut blocks, narrowed to fill the device on small lists
   if (K == 1 && (!useN || p->aliased) && !p->partT[0].rowsInKey && p->nnzT > 0) {
     const Partition *vp = vec_partition<T>(p, 1, 0);
-    if (vp && vp->nblocks < 16 * p->cus && vp + 1 < p->partT + p->nparts && vp + 1 < p->partT + kPackParts) ++vp;
+    if (vp) vp = fill_device(p, vp);
     if (vp) {
       const ListView G = list_view(p, false, vp, false);
       const T *v = static_cast<const T *>(d_v);
@@ -1883,12 +1895,12 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const bool partN = useN && !p->aliased;
   // K = 1 from a T list without empty rows: rows staged in LDS, each entry's
   // row found by counting heads, on the merge-path blocks (measured faster
-  // for this kernel than blocks cut on entries, except at degree 1); when
-  // those leave fewer than 16 workgroups per CU, on the entry-cut blocks one
-  // step narrower (more blocks, with a pack)
+  // for this kernel than blocks cut on entries, except at degree 1); on a
+  // small list, on the narrower entry-cut blocks (with packs) that fill the
+  // device
   if (K == 1 && !partN && !p->partT[0].rowsInKey && p->nnzT > 0 && p->partS[0].maxT <= kCap) {
     const Partition *vp = &p->partS[0];
-    if (vp->nblocks < 16 * p->cus && p->nparts > 1) vp = &p->partT[1];
+    if (vp->nblocks < kFillPerCu * p->cus && p->nparts > 1) vp = fill_device(p, &p->partT[1]);
     const ListView S = list_view(p, false, vp, false);
     hipLaunchKernelGGL((k_scatter_vec<T>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
                        S.base, S.stride, static_cast<const T *>(d_gv), static_cast<T *>(d_v),
