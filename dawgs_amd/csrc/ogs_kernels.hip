@@ -6,12 +6,14 @@
 // gather list in CSR order starting from the op's identity, then write the
 // result to the row's scatter list.  How they compute it is MI355X-first:
 //
-//  * Partition.  A plan cuts the rows into blocks by a merge-path rule: row r
-//    belongs to block floor((rowStarts[r] + r) / W).  Every block then holds
-//    at most W rows and at most W + Lmax - 1 nonzeros, so the per-block LDS
-//    staging never overflows, and the cut is computed on the device by
-//    binary search (no host pass over the CSR).  Blocks are contiguous row
-//    ranges, so each block's column set is a compact window of v.
+//  * Partition.  A plan cuts the rows into blocks of contiguous rows: for a
+//    list without empty rows row r belongs to block floor(rowStarts[r] / W)
+//    (at most W + Lmax - 1 nonzeros, so no more rows than that either); for
+//    a list with empty rows, and for the scatter kernels, by the merge-path
+//    rule floor((rowStarts[r] + r) / W) (at most W rows and W + Lmax - 1
+//    nonzeros).  Either way the per-block LDS staging never overflows, and
+//    the cut is computed on the device by binary search.  Launches on small
+//    lists take narrower blocks until the device is filled.
 //  * Packs.  For the partitions a plan serves, each block's list entries are
 //    sorted by column at plan creation and stored as 32-bit words (column
 //    segment and offset, row-head bit, CSR position) at a fixed stride per
@@ -31,10 +33,14 @@
 //    accumulate in float with a two-sum running error.
 //  * XCDs.  Workgroups are dispatched round-robin over the 8 XCDs; block ids
 //    are remapped so that each XCD walks one contiguous eighth of the rows
-//    and neighbouring blocks share that XCD's L2.
+//    and neighbouring blocks share that XCD's L2.  Where rows sharing cache
+//    lines sit a mesh layer of blocks apart (low degree), a plan may walk
+//    each XCD's blocks in tiles of layers instead (kept only where measured
+//    faster at plan creation).
 //  * K fields.  Fields are interleaved (v[k + id*K]).  For K in {2, 3, 4, 8,
-//    16} a work item is up to 4 consecutive fields of one entry (one index
-//    word feeds contiguous value loads); other K run in field chunks.  The
+//    16} a work item is up to 4 consecutive fields of one entry (8-byte
+//    values: 2, or 1 at K = 3), so one index word feeds contiguous value
+//    loads; other K run in field chunks.  The
 //    indices are read once per block rather than once per field as in the
 //    reference (ogsKernels.okl:41, @outer(1) over K).
 //
