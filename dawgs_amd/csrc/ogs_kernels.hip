@@ -1039,12 +1039,66 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
 }
 
-// Gather, K = 1, of a T list without empty rows: the GatherScatter body up to
-// the row reduction (values gathered through the column-sorted pack and
-// staged at their CSR positions, rows found from the head bits, so no row
-// starts are read), each row's result stored at its row -- the number of row
-// heads at or before its head's position -- instead of scattered back.
-template <typename T, int OP>
+// Exclusive prefix counts of the head-mask words (one wave): pc[w] = heads
+// in positions [0, 32 w).  An entry's row within its block is then the
+// number of heads at or before its CSR position, minus one (head_row).
+__device__ __forceinline__ void head_prefix(const uint32_t *__restrict__ hm, int n, int32_t *__restrict__ pc) {
+  constexpr int NW = kCap / 32;
+  static_assert(NW == 64, "one wave scans the head-mask words");
+  const int tid = threadIdx.x;
+  if (tid < NW) {
+    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < NW; d <<= 1) {
+      const int y = __shfl_up(incl, d, NW);
+      if (tid >= d) incl += y;
+    }
+    pc[tid] = incl - c;
+  }
+}
+__device__ __forceinline__ int head_row(const uint32_t *__restrict__ hm, const int32_t *__restrict__ pc, int ps) {
+  const int wd = ps >> 5;
+  const uint32_t upto = (uint32_t)((2ull << (ps & 31)) - 1u);  // positions 32 wd .. ps
+  return pc[wd] + __popc(hm[wd] & upto) - 1;
+}
+
+// Row heads of a block as bytes at CSR positions: from the pack words, or
+// from the row starts for a block kept in CSR order (no empty rows).
+template <int SL>
+__device__ __forceinline__ void stage_heads(const VecStage<SL> &s, const VecMeta &m, int C,
+                                            const int32_t *__restrict__ starts, uint8_t *__restrict__ head) {
+  const int tid = threadIdx.x;
+  const int ni = s.n * C;
+  const bool packed = s.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < ni && it % C == 0) {
+      if (packed) head[s.w[j] & kPackPosMask] = (uint8_t)((s.w[j] >> kPackHeadBit) & 1u);
+      else head[it / C] = 0;
+    }
+  }
+  if (!packed) {
+    const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;
+    int rg[kRowSlots];
+    row_words(starts, r0, nr, rg);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRowSlots; ++j) {
+      const int r = tid + j * kBlock;
+      if (r < nr) head[rg[j] - s.g0] = 1;
+    }
+  }
+}
+
+// Gather of a T list without empty rows, KT interleaved fields (compile-time
+// shape): the GatherScatter body up to the row reduction (values gathered
+// through the column-sorted pack and staged at their CSR positions, rows
+// found from the head bits, so no row starts are read), each (row, field
+// chunk)'s result stored at its row -- the number of row heads at or before
+// its head's position -- instead of scattered back.
+template <typename T, int OP, int KT>
 __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
                                                        const int32_t *__restrict__ starts,
                                                        const int32_t *__restrict__ ids,
@@ -1052,91 +1106,72 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
                                                        const int4 *__restrict__ base, int stride,
                                                        const T *__restrict__ q, T *__restrict__ gq,
                                                        const int *__restrict__ order) {
-  using V = VecShape<T, 1>;
+  using V = VecShape<T, KT>;
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
-  constexpr int SL = V::SL;
-  constexpr int NW = kCap / 32;
-  static_assert(NW == 64, "one wave scans the head-mask words");
-  static_assert(V::CAP == kCap, "K = 1 blocks hold kCap entries");
+  constexpr int SL = V::SL, VW = V::VW, C = V::C;
   __shared__ L val[vec_cap<L>()];
   __shared__ uint32_t head4[kCap / 4];
-  __shared__ uint32_t hm[NW + 1];
-  __shared__ int32_t pc[NW];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
   VecStage<SL> s;
   VecMeta m;
-  T x[SL][1];
+  T x[SL][VW];
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   const int b = order ? order[xb] : xb;
   vec_meta(src, b, m);
-  vec_index<T, 1>(s, src, b, m);
-  vec_gather<T, 1>(s, x);
+  vec_index<T, KT>(s, src, b, m);
+  vec_gather<T, KT>(s, x);
   const int n = s.n;
   if (n == 0) return;
+  const int ni = n * C;
   const int tid = threadIdx.x;
-  const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;
+  const int r0 = uniform(m.p0.x);
   const bool packed = s.bb.x >= 0;
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
     const int it = tid + j * kBlock;
-    if (it < n) {
-      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it;
-      val[ps] = Lds<T>::in(x[j][0]);
-      head[ps] = packed ? (uint8_t)((s.w[j] >> kPackHeadBit) & 1u) : (uint8_t)0;
-    }
-  }
-  if (!packed) {
-    int rg[kRowSlots];
-    row_words(starts, r0, nr, rg);
-    __syncthreads();
+    if (it < ni) {
+      const int c = it % C;
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
 #pragma unroll
-    for (int j = 0; j < kRowSlots; ++j) {
-      const int r = tid + j * kBlock;
-      if (r < nr) head[rg[j] - s.g0] = 1;
+      for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = Lds<T>::in(x[j][v]);
     }
   }
+  stage_heads(s, m, C, starts, head);
   __syncthreads();
   head_mask(head, n, hm);
   __syncthreads();
-  if (tid < NW) {  // exclusive prefix counts of the heads, word by word
-    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
-    int incl = c;
-#pragma unroll
-    for (int d = 1; d < NW; d <<= 1) {
-      const int y = __shfl_up(incl, d, NW);
-      if (tid >= d) incl += y;
-    }
-    pc[tid] = incl - c;
-  }
+  head_prefix(hm, n, pc);
   __syncthreads();
-  for (int i = tid; i < n; i += kBlock) {
-    const int wd = i >> 5;
-    const uint32_t bits = hm[wd];
-    if (!((bits >> (i & 31)) & 1u)) continue;
+  for (int t = tid; t < ni; t += kBlock) {
+    const int i = t / C, c = t - i * C;
+    if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
     const int len = row_len(hm, i);
-    typename R::A red = R::init();
-    int k = 0;
-    for (; k + 4 <= len; k += 4) {  // CSR order, members read four at a time
-      L y[4];
+    typename R::A red[VW];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) y[u] = val[i + k + u];
+    for (int v = 0; v < VW; ++v) red[v] = R::init();
+    const L *sv = val + i * KT + c * VW;
+    // CSR order, one member at a time (reading four at a time, as the
+    // GatherScatter does, more than doubles this kernel's registers)
+    for (int k = 0; k < len; ++k)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) R::apply(red, y[u]);
-    }
-    for (; k < len; ++k) R::apply(red, val[i + k]);
-    const int row = pc[wd] + __popc(bits & (uint32_t)((2ull << (i & 31)) - 1u)) - 1;
-    gq[r0 + row] = Lds<T>::out(R::fin(red));
+      for (int v = 0; v < VW; ++v) R::apply(red[v], sv[k * KT + v]);
+    T *dst = gq + (size_t)(r0 + head_row(hm, pc, i)) * KT + c * VW;
+#pragma unroll
+    for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(R::fin(red[v]));
   }
 }
 
-// Scatter, K = 1, of a T list without empty rows over an entry-cut
-// partition: the block's row values are staged in LDS; an entry's row is the
-// number of row heads at or before its CSR position (the head bit mask and
-// per-word prefix counts), and each entry stores its row's value through its
-// pack word (or in CSR order for a block the pack could not encode).
-template <typename T>
+// Scatter of a T list without empty rows, KT interleaved fields: the block's
+// rows (KT values each, contiguous) are staged in LDS; an item's row is the
+// number of row heads at or before its entry's CSR position (the head bit
+// mask and per-word prefix counts), and each item stores its row's fields
+// through its pack word (or in CSR order for a block the pack could not
+// encode).
+template <typename T, int KT>
 __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__ part,
                                                         const int32_t *__restrict__ starts,
                                                         const int32_t *__restrict__ ids,
@@ -1144,13 +1179,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__
                                                         const int4 *__restrict__ base, int stride,
                                                         const T *__restrict__ gq, T *__restrict__ q,
                                                         const int *__restrict__ order) {
-  constexpr int SL = kSlots;  // kCap entries and at most kCap rows per block
-  constexpr int NW = kCap / 32;
-  static_assert(NW == 64, "one wave scans the head-mask words");
-  __shared__ T srow[kCap];
+  using V = VecShape<T, KT>;
+  constexpr int SL = V::SL, VW = V::VW, C = V::C;
+  // a block's rows hold at most CAP * KT values (no more rows than entries)
+  constexpr int RV = V::CAP * KT, RS = (RV + kBlock - 1) / kBlock;
+  static_assert(RV * (int)sizeof(T) <= kVecBytes, "a block's rows fit the LDS stage");
+  __shared__ T srow[RV];
   __shared__ uint32_t head4[kCap / 4];
-  __shared__ uint32_t hm[NW + 1];
-  __shared__ int32_t pc[NW];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const VecSrc src = {part, ids, pk, base, q, 0, stride};
   VecStage<SL> s;
@@ -1158,60 +1195,41 @@ __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   const int b = order ? order[xb] : xb;
   vec_meta(src, b, m);
-  vec_index<T, 1>(s, src, b, m);
+  vec_index<T, KT>(s, src, b, m);
   const int n = s.n;
   if (n == 0) return;
+  const int ni = n * C;
   const int tid = threadIdx.x;
-  const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;  // nr <= n: no empty rows
-  T x[SL];
+  const int r0 = uniform(m.p0.x), nv = (uniform(m.p1.x) - r0) * KT;  // nr <= n: no empty rows
+  const T *g = gq + (size_t)r0 * KT;
+  T y[RS];
 #pragma unroll
-  for (int j = 0; j < SL; ++j) {
+  for (int j = 0; j < RS; ++j) {
     const int r = tid + j * kBlock;
-    x[j] = r < nr ? gq[r0 + r] : T{};
+    y[j] = r < nv ? g[r] : T{};
   }
+#pragma unroll
+  for (int j = 0; j < RS; ++j) {
+    const int r = tid + j * kBlock;
+    if (r < nv) srow[r] = y[j];
+  }
+  stage_heads(s, m, C, starts, head);
+  __syncthreads();
+  head_mask(head, n, hm);
+  __syncthreads();
+  head_prefix(hm, n, pc);
+  __syncthreads();
   const bool packed = s.bb.x >= 0;
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
     const int it = tid + j * kBlock;
-    if (it < nr) srow[it] = x[j];
-    if (it < n) {
-      if (packed) head[s.w[j] & kPackPosMask] = (uint8_t)((s.w[j] >> kPackHeadBit) & 1u);
-      else head[it] = 0;
-    }
-  }
-  if (!packed) {
-    int rg[kRowSlots];
-    row_words(starts, r0, nr, rg);
-    __syncthreads();
+    if (it < ni) {
+      const int c = it % C;
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
+      const T *sv = srow + head_row(hm, pc, ps) * KT + c * VW;
+      T *d = q + (size_t)slot_col(s.w[j], s.bb, packed) * KT + c * VW;
 #pragma unroll
-    for (int j = 0; j < kRowSlots; ++j) {
-      const int r = tid + j * kBlock;
-      if (r < nr) head[rg[j] - s.g0] = 1;
-    }
-  }
-  __syncthreads();
-  head_mask(head, n, hm);
-  __syncthreads();
-  if (tid < NW) {  // exclusive prefix counts of the heads, word by word
-    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
-    int incl = c;
-#pragma unroll
-    for (int d = 1; d < NW; d <<= 1) {
-      const int y = __shfl_up(incl, d, NW);
-      if (tid >= d) incl += y;
-    }
-    pc[tid] = incl - c;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < SL; ++j) {
-    const int it = tid + j * kBlock;
-    if (it < n) {
-      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it;
-      const int wd = ps >> 5;
-      const uint32_t upto = (uint32_t)((2ull << (ps & 31)) - 1u);  // positions 32 wd .. ps
-      const int row = pc[wd] + __popc(hm[wd] & upto) - 1;
-      q[slot_col(s.w[j], s.bb, packed)] = srow[row];
+      for (int v = 0; v < VW; ++v) d[v] = sv[v];
     }
   }
 }
@@ -1833,35 +1851,72 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   return (int)hipGetLastError();
 }
 
+template <typename T, int OP>
+static int launch_gather_vec(int K, const Partition *vp, const ListView &G, const Plan *p, const T *v, T *gv,
+                             const int *order, hipStream_t st) {
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL((k_gather_vec<T, OP, KT>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT,
+                       G.pk, G.base, G.stride, v, gv, order);
+  };
+  switch (K) {
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    default: go(std::integral_constant<int, 16>{}); break;
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int launch_scatter_vec(int K, const Partition *vp, const ListView &S, const Plan *p, const T *gv, T *v,
+                              hipStream_t st) {
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL((k_scatter_vec<T, KT>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
+                       S.base, S.stride, gv, v, nullptr);
+  };
+  switch (K) {
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    default: go(std::integral_constant<int, 16>{}); break;
+  }
+  return (int)hipGetLastError();
+}
+
 template <typename T>
 static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_gv, const void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:71-79
   const bool useN = (trans == OGS_NOTRANS);
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
-  // K = 1 from a T list without empty rows: the head-counting kernel on the
-  // entry-cut blocks, narrowed to fill the device on small lists
-  if (K == 1 && (!useN || p->aliased) && !p->partT[0].rowsInKey && p->nnzT > 0) {
-    const Partition *vp = vec_partition<T>(p, 1, 0);
-    if (vp) vp = fill_device(p, vp);
+  // a field count with a compile-time shape from a T list without empty
+  // rows: the head-counting kernel on the entry-cut blocks whose blocks fit
+  // the shape, narrowed to fill the device on small lists
+  if (vec_k(K) && (!useN || p->aliased) && !p->partT[0].rowsInKey && p->nnzT > 0) {
+    const Partition *vp = vec_partition<T>(p, K, 0);
     if (vp) {
+      vp = fill_device(p, vp);
       const ListView G = list_view(p, false, vp, false);
-      const T *v = static_cast<const T *>(d_v);
-      T *gv = static_cast<T *>(d_gv);
-#define OGS_GV_LAUNCH(OPV)                                                                                     \
-  hipLaunchKernelGGL((k_gather_vec<T, OPV>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, G.pk, \
-                     G.base, G.stride, v, gv, (vp->tiled & kTiledGather) ? vp->order : nullptr);
+      const int *order = (K == 1 && (vp->tiled & kTiledGather)) ? vp->order : nullptr;
       switch (op) {
-        case OGS_ADD: OGS_GV_LAUNCH(OGS_ADD) break;
+        case OGS_ADD: return launch_gather_vec<T, OGS_ADD>(K, vp, G, p, static_cast<const T *>(d_v),
+                                                           static_cast<T *>(d_gv), order, st);
         case OGS_MUL:
-          if constexpr (kOffered<T, OGS_MUL>) { OGS_GV_LAUNCH(OGS_MUL) } else { return OGS_E_NOT_SUPPORTED; }
-          break;
-        case OGS_MAX: OGS_GV_LAUNCH(OGS_MAX) break;
-        case OGS_MIN: OGS_GV_LAUNCH(OGS_MIN) break;
+          if constexpr (kOffered<T, OGS_MUL>)
+            return launch_gather_vec<T, OGS_MUL>(K, vp, G, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv),
+                                                 order, st);
+          else return OGS_E_NOT_SUPPORTED;
+        case OGS_MAX: return launch_gather_vec<T, OGS_MAX>(K, vp, G, p, static_cast<const T *>(d_v),
+                                                           static_cast<T *>(d_gv), order, st);
+        case OGS_MIN: return launch_gather_vec<T, OGS_MIN>(K, vp, G, p, static_cast<const T *>(d_v),
+                                                           static_cast<T *>(d_gv), order, st);
         default: return OGS_E_ARG;
       }
-#undef OGS_GV_LAUNCH
-      return (int)hipGetLastError();
     }
   }
   const bool partN = useN && !p->aliased;
@@ -1908,10 +1963,21 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
     const Partition *vp = &p->partS[0];
     if (vp->nblocks < kFillPerCu * p->cus && p->nparts > 1) vp = fill_device(p, &p->partT[1]);
     const ListView S = list_view(p, false, vp, false);
-    hipLaunchKernelGGL((k_scatter_vec<T>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
+    hipLaunchKernelGGL((k_scatter_vec<T, 1>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
                        S.base, S.stride, static_cast<const T *>(d_gv), static_cast<T *>(d_v),
                        (vp->tiled & kTiledScatter) ? vp->order : nullptr);
     return (int)hipGetLastError();
+  }
+  // other field counts with a compile-time shape: the same kernel on the
+  // entry-cut blocks whose blocks fit the shape (with packs), narrowed to
+  // fill the device on small lists
+  if (vec_k(K) && !partN && !p->partT[0].rowsInKey && p->nnzT > 0) {
+    const Partition *vp = vec_partition<T>(p, K, 0);
+    if (vp) {
+      vp = fill_device(p, vp);
+      return launch_scatter_vec<T>(K, vp, list_view(p, false, vp, false), p, static_cast<const T *>(d_gv),
+                                   static_cast<T *>(d_v), st);
+    }
   }
   const Partition *parts = partN ? p->partN : p->partS;
   int kc = 0;
@@ -2106,7 +2172,7 @@ static int tile_schedule(Plan *p, hipStream_t st) {
       }, oT, st, e0, e1, &gs);
     if (!rc)
       rc = order_pays([&](const int *o) {
-        hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
+        hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
                            p->ciT, k.pk, k.base, k.stride, vals, rows, o);
       }, oT, st, e0, e1, &ga);
   }
@@ -2115,7 +2181,7 @@ static int tile_schedule(Plan *p, hipStream_t st) {
     const Pack &k = p->packS;
     if (!rc)
       rc = order_pays([&](const int *o) {
-        hipLaunchKernelGGL((k_scatter_vec<double>), dim3(ps.nblocks), dim3(kBlock), 0, st, ps.d, p->rsT, p->ciT,
+        hipLaunchKernelGGL((k_scatter_vec<double, 1>), dim3(ps.nblocks), dim3(kBlock), 0, st, ps.d, p->rsT, p->ciT,
                            k.pk, k.base, k.stride, rows, vals, o);
       }, oS, st, e0, e1, &sc);
   }
