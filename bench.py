@@ -91,12 +91,12 @@ def host_lines(res, log) -> dict:
     from dawgs_amd.perf import line
     from oracle import oracle as O
     ogs, sogs = res["_ogs"], res["_sogs"]
-    N, Ng, p = res["N"], res["NgatherGlobal"], res["p"]
+    N, Ng, p, K = res["N"], res["NgatherGlobal"], res["p"], res["K"]
     c = ogs.csr()
     u = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=N)
     c = sogs.csr()
     sg = O.OracleCsr(c["rowStartsT"], c["colIdsT"], c["rowStartsN"], c["colIdsN"], sogs.NrowsN, N)
-    v = np.random.default_rng(2).uniform(-1, 1, N)
+    v = np.random.default_rng(2).uniform(-1, 1, N * K)
     out = {}
 
     def timed(fn, n=10):
@@ -106,14 +106,14 @@ def host_lines(res, log) -> dict:
             fn()
         return (time.perf_counter() - t0) / n
 
-    t = timed(lambda: u.gather_scatter(v, 1, O.ADD, O.SYM))
+    t = timed(lambda: u.gather_scatter(v, K, O.ADD, O.SYM))
     log(line("Host  ", "GatherScatter", Ng, N, p, t, res["gs"]["bytes"]))
     out["gs"] = t
-    gv = sg.gather(v, 1, O.ADD, O.TRANS)
-    t = timed(lambda: sg.gather(v, 1, O.ADD, O.TRANS))
+    gv = sg.gather(v, K, O.ADD, O.TRANS)
+    t = timed(lambda: sg.gather(v, K, O.ADD, O.TRANS))
     log(line("Host  ", "Gather", Ng, N, p, t, res["gather"]["bytes"]))
     out["gather"] = t
-    t = timed(lambda: sg.scatter(v, gv, 1, O.NOTRANS))
+    t = timed(lambda: sg.scatter(v, gv, K, O.NOTRANS))
     log(line("Host  ", "Scatter", Ng, N, p, t, res["scatter"]["bytes"]))
     out["scatter"] = t
     return out
@@ -129,7 +129,7 @@ def run_sweep(args) -> int:
     for p in [int(x) for x in args.sweep.split(",")]:
         nx = SWEEP_NX[p]
         log(f" ---- degree {p}: {nx}^3 elements ----")
-        res = performance_test(nx, nx, nx, p, warmup=args.warmup, steps=args.steps, log=log)
+        res = performance_test(nx, nx, nx, p, K=args.k, warmup=args.warmup, steps=args.steps, log=log)
         row = {k: v for k, v in res.items() if not k.startswith("_")}
         if not args.no_cpu:
             row["host_s"] = host_lines(res, log)
@@ -137,6 +137,7 @@ def run_sweep(args) -> int:
         del res
         gc.collect()
     out = {"metric": "ogs degree sweep (Device/Host GatherScatter, Gather, Scatter; fp64 Add; ~56M local DOFs)",
+           "fields": args.k,
            "unit": "DOFS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
            "data": "synthetic", "host_threads": O.num_threads() if not args.no_cpu else None,
            "host_cpu": host_cpu() if not args.no_cpu else None,
@@ -401,6 +402,7 @@ def main() -> int:
                     help="degree sweep at ~56 M local DOFs (48^3 at p=7): Device (and, unless --no-cpu, "
                          "Host) GatherScatter/Gather/Scatter lines in the reference's format; one JSON "
                          "summary line on stdout")
+    ap.add_argument("--k", type=int, default=1, help="fields per node (interleaved) for --sweep")
     ap.add_argument("--batched", action="store_true",
                     help="batched form: 48 meshes (nx,ny,nz ~ U{4..32}, p ~ U{1..7}, K ~ {1,3,8,16}), fp64 "
                          "Add GatherScatter in one persistent graph-captured launch vs one plain launch per "
