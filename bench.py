@@ -19,7 +19,7 @@ oracle) and `batched` (48 meshes in one graph launch against one launch per
 mesh and field); --no-extras skips them.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 100] [--nx 48] [--p 7] [--no-cpu] [--copy-probe]
-    python bench.py --sweep [DEGREES] | --batched | --precision | --readme
+    python bench.py --sweep [DEGREES] [--k K] | --batched | --precision | --readme
 """
 from __future__ import annotations
 
