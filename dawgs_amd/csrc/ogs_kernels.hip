@@ -1234,6 +1234,103 @@ __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__
   }
 }
 
+// GatherScatter with two lists (NoTrans / Trans of an operator whose N and T
+// lists differ), KT interleaved fields (compile-time shape), over an
+// entry-cut T partition whose blocks hold at most CAP/2 entries of either
+// list, so that the gathered values, the rows' results and the row starts
+// share one list's LDS budget.  Gather side: items gathered through the
+// list's pack (or in CSR order) and staged at their CSR positions; one lane
+// per (row, field chunk) reduces the row in CSR order from the row starts (an
+// empty row gives the op's identity, as on the host).  Scatter side: a map
+// from CSR position to row, from the scatter list's row starts; each item
+// stores its row's result through its pack word (or in CSR order).
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs2_vec(const int4 *__restrict__ part, ListView G, int gUseN, ListView S,
+                                                    int sUseN, T *__restrict__ q) {
+  using V = VecShape<T, KT>;
+  using L = typename Lds<T>::L;
+  using R = LRed<T, OP>;
+  constexpr int SL = V::SL, VW = V::VW, C = V::C;
+  constexpr int CAP2 = V::CAP / 2;
+  constexpr int RSL = (CAP2 + 1 + kBlock - 1) / kBlock;
+  __shared__ L val[CAP2 * KT];
+  __shared__ L rowv[CAP2 * KT];  // no more rows than T entries
+  __shared__ int32_t rsg[CAP2 + 1], rss[CAP2 + 1];
+  __shared__ int32_t srow[CAP2];
+  const VecSrc gsrc = {part, G.ids, G.pk, G.base, q, gUseN, G.stride};
+  const VecSrc ssrc = {part, S.ids, S.pk, S.base, q, sUseN, S.stride};
+  VecStage<SL> gs, ss;
+  VecMeta m, ms;
+  T x[SL][VW];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  vec_meta(gsrc, b, m);
+  vec_meta(ssrc, b, ms);
+  vec_index<T, KT>(gs, gsrc, b, m);
+  vec_index<T, KT>(ss, ssrc, b, ms);
+  vec_gather<T, KT>(gs, x);
+  const int tid = threadIdx.x;
+  const int r0 = uniform(m.p0.x), nr = uniform(m.p1.x) - r0;
+  int rg[RSL], rv[RSL];
+#pragma unroll
+  for (int j = 0; j < RSL; ++j) {
+    const int r = tid + j * kBlock;
+    rg[j] = r <= nr ? G.starts[r0 + r] - gs.g0 : 0;
+    rv[j] = r <= nr ? S.starts[r0 + r] - ss.g0 : 0;
+  }
+  const int ng = gs.n * C;
+  const bool gp = gs.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < ng) {
+      const int c = it % C;
+      const int ps = gp ? (int)(gs.w[j] & kPackPosMask) : it / C;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = Lds<T>::in(x[j][v]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < RSL; ++j) {
+    const int r = tid + j * kBlock;
+    if (r <= nr) {
+      rsg[r] = rg[j];
+      rss[r] = rv[j];
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < nr; r += kBlock) {
+    const int z = rss[r + 1];
+    for (int i = rss[r]; i < z; ++i) srow[i] = r;
+  }
+  for (int t = tid; t < nr * C; t += kBlock) {
+    const int r = t / C, c = t - r * C;
+    const int z = rsg[r + 1];
+    typename R::A red[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[v] = R::init();
+    for (int i = rsg[r]; i < z; ++i)
+#pragma unroll
+      for (int v = 0; v < VW; ++v) R::apply(red[v], val[i * KT + c * VW + v]);
+#pragma unroll
+    for (int v = 0; v < VW; ++v) rowv[r * KT + c * VW + v] = R::fin(red[v]);
+  }
+  __syncthreads();
+  const int ns = ss.n * C;
+  const bool sp = ss.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < ns) {
+      const int c = it % C;
+      const int ps = sp ? (int)(ss.w[j] & kPackPosMask) : it / C;
+      const L *sv = rowv + srow[ps] * KT + c * VW;
+      T *d = q + (size_t)slot_col(ss.w[j], ss.bb, sp) * KT + c * VW;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) d[v] = Lds<T>::out(sv[v]);
+    }
+  }
+}
+
 // Batched form: one (descriptor, block) item per workgroup.
 template <typename T, int OP, int KT>
 __device__ __forceinline__ void batch_item(const BatchDev *__restrict__ descs, int2 w,
@@ -1797,6 +1894,36 @@ static int occupancy_of(F kernel) {
   return n;
 }
 
+// The widest entry-cut T partition whose blocks hold at most half the shape's
+// entries of either list (the two-list kernel's capacity).
+template <typename T>
+static const Partition *vec2_partition(const Plan *p, int K) {
+  const int cap = vec_cap_entries<T>(K) / 2;
+  for (int j = 0; j < p->nparts; ++j) {
+    const int m = p->partT[j].maxT > p->partT[j].maxN ? p->partT[j].maxT : p->partT[j].maxN;
+    if (m <= cap) return &p->partT[j];
+  }
+  return nullptr;
+}
+
+template <typename T, int OP>
+static int launch_gs2(int K, const Partition *vp, const ListView &G, int gN, const ListView &S, int sN, T *v,
+                      hipStream_t st) {
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL((k_gs2_vec<T, OP, KT>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, G, gN, S, sN, v);
+  };
+  switch (K) {
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    default: go(std::integral_constant<int, 16>{}); break;
+  }
+  return (int)hipGetLastError();
+}
+
 template <typename T>
 static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:321-339
@@ -1830,6 +1957,25 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
     const Partition *vp = vec_partition<T>(p, K, 0);
     if (vp) vp = fill_device(p, vp);
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
+  }
+  // two lists, K > 1 with a compile-time shape, T without empty rows: the
+  // two-list entry-major kernel (at K = 1 the row-based kernel below, with a
+  // whole list's capacity per block, measured 7-13% faster)
+  if (!sym && K > 1 && vec_k(K) && !p->partT[0].rowsInKey && p->nnzT > 0) {
+    const Partition *vp = vec2_partition<T>(p, K);
+    if (vp) {
+      vp = fill_device(p, vp);
+      const ListView G2 = list_view(p, gListN, vp, false), S2 = list_view(p, sListN, vp, false);
+      switch (op) {
+        case OGS_ADD: return launch_gs2<T, OGS_ADD>(K, vp, G2, gN, S2, sN, v, st);
+        case OGS_MUL:
+          if constexpr (kOffered<T, OGS_MUL>) return launch_gs2<T, OGS_MUL>(K, vp, G2, gN, S2, sN, v, st);
+          else return OGS_E_NOT_SUPPORTED;
+        case OGS_MAX: return launch_gs2<T, OGS_MAX>(K, vp, G2, gN, S2, sN, v, st);
+        case OGS_MIN: return launch_gs2<T, OGS_MIN>(K, vp, G2, gN, S2, sN, v, st);
+        default: return OGS_E_ARG;
+      }
+    }
   }
 #define OGS_GS_LAUNCH(OPV)                                                                                      \
   if (K == 1 && !sym) {                                                                                         \
