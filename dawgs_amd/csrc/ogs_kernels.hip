@@ -2182,9 +2182,10 @@ static int scan_csr(const int32_t *rs, int nrows, const int32_t *ci, int nnz, in
   return OGS_SUCCESS;
 }
 
-// Block order of partition `pt` for layer distance D: each XCD's run of
-// blocks (the runs of xcd_block) walked in bands of kTileL layers of D blocks,
-// each band in tiles of kTileP blocks, each tile layer by layer.
+// Block order of a partition of n blocks for layer distance D: each XCD's
+// run of blocks (the runs of xcd_block) walked in bands of kTileL layers of D
+// blocks, each band in tiles of kTileP blocks, each tile layer by layer;
+// order[slot] = block, a permutation of [0, n).
 static std::vector<int> tiled_order(int n, int D) {
   std::vector<int> order(n);
   const int q = n / kNumXcd, r = n % kNumXcd;
