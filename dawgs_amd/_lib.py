@@ -19,7 +19,7 @@ _I32P = C.POINTER(C.c_int32)
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
-                 "packedK1", "plainK1", "Ncols", "tileD")]
+                 "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN")]
 
 
 class BatchDesc(C.Structure):

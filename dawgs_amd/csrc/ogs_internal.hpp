@@ -75,9 +75,10 @@ struct ogs_hip_plan_s {
   int nparts = 0;
   ogs_hip::Partition partT[ogs_hip::kMaxParts];  // cut on T, rows [0, NrowsT)
   ogs_hip::Partition partN[ogs_hip::kMaxParts];  // cut on N, rows [0, NrowsN)
-  // T-list partitions of the scatter kernels: merge path (measured faster
-  // for them than blocks cut on entries); copies of partT when those are
-  // merge path (a T list with empty rows)
+  // T-list partitions of the row-wise scatter kernels: merge path (measured
+  // faster for them than blocks cut on entries); copies of partT when those
+  // are merge path (a T list with empty rows) or when the T list scatters
+  // by columns (colRowT)
   ogs_hip::Partition partS[ogs_hip::kMaxParts];
   // column-sorted packs: the T list over partT[0..kPackParts) (K=1 and the
   // K-field Sym kernels), the N list over partT[0] and over partN[0] (K=1)
@@ -85,6 +86,11 @@ struct ogs_hip_plan_s {
   ogs_hip::Pack packNT;
   ogs_hip::Pack packNN;
   ogs_hip::Pack packS;  // the T list over partS[0] (a copy of packT[0] when partS is partT)
+  // per column, the largest row of the list holding it (-1: none), for lists
+  // with at least half as many entries as columns (else null); colRowN is
+  // colRowT for an Unsigned operator
+  int32_t *colRowT = nullptr;
+  int32_t *colRowN = nullptr;
 };
 
 namespace ogs_hip {

@@ -1238,6 +1238,63 @@ __global__ __launch_bounds__(kBlock) void k_scatter_vec(const int4 *__restrict__
   }
 }
 
+// Scatter by columns: v[c*K + k] = gv[colRow[c]*K + k] for every column c
+// with a row (colRow[c] >= 0: the largest row of the list that holds c; a
+// list built by Setup holds each column at most once), columns without a row
+// untouched.  Workgroup b covers columns [b*cpb, (b+1)*cpb), whose values
+// (at most kColVals) are contiguous, so the stores stream whole lines in
+// order; each lane loads its columns' rows (non-temporal), then their values,
+// then stores.  KT > 0: K = KT at compile time; KT = 0: any K (cpb = 1 past
+// kColVals fields).
+constexpr int kColVals = 2048;  // values per workgroup (8 per lane)
+
+template <typename T, int KT>
+__global__ __launch_bounds__(kBlock) void k_scatter_cols(int32_t ncols, int32_t K, int32_t cpb,
+                                                         const int32_t *__restrict__ colRow,
+                                                         const T *__restrict__ gv, T *__restrict__ v) {
+  constexpr int U = kColVals / kBlock;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c0 = (int64_t)b * cpb;
+  const int64_t left = (int64_t)ncols - c0;
+  const int nc = left < cpb ? (int)left : cpb;
+  const int tid = threadIdx.x;
+  if constexpr (KT > 0) {
+    const int nv = nc * KT;
+    int r[U], k[U];
+    int64_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = tid + u * kBlock;
+      const int cl = t / KT;
+      k[u] = t - cl * KT;
+      c[u] = c0 + cl;
+      r[u] = t < nv ? __builtin_nontemporal_load(colRow + c[u]) : -1;
+    }
+    T x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] >= 0) x[u] = gv[(size_t)r[u] * KT + k[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] >= 0) v[(size_t)c[u] * KT + k[u]] = x[u];
+  } else {
+    const int64_t nv = (int64_t)nc * K;
+    for (int64_t t = tid; t < nv; t += kBlock) {
+      const int64_t cl = t / K, kk = t - cl * K;
+      const int r = colRow[c0 + cl];
+      if (r >= 0) v[(size_t)(c0 + cl) * K + kk] = gv[(size_t)r * K + kk];
+    }
+  }
+}
+
+// colRow[c] = the largest row of rows [0, nrows) whose list holds column c
+// (colRow set to -1 beforehand), one lane per row.
+__global__ __launch_bounds__(kBlock) void k_col_rows(const int32_t *__restrict__ rs, int32_t nrows,
+                                                     const int32_t *__restrict__ ci, int32_t *__restrict__ colRow) {
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kBlock)
+    for (int j = rs[r]; j < rs[r + 1]; ++j) atomicMax(colRow + ci[j], (int32_t)r);
+}
+
 // GatherScatter with two lists (NoTrans / Trans of an operator whose N and T
 // lists differ), KT interleaved fields (compile-time shape), over an
 // entry-cut T partition whose blocks hold at most CAP/2 entries of either
@@ -2047,6 +2104,27 @@ static int launch_scatter_vec(int K, const Partition *vp, const ListView &S, con
 }
 
 template <typename T>
+static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, const T *gv, T *v, hipStream_t st) {
+  if (ncols == 0) return OGS_SUCCESS;
+  const int32_t cpb = K <= kColVals ? kColVals / K : 1;
+  const dim3 grid((unsigned)(((int64_t)ncols + cpb - 1) / cpb)), block(kBlock);
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL((k_scatter_cols<T, KT>), grid, block, 0, st, ncols, K, cpb, colRow, gv, v);
+  };
+  switch (K) {
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 16: go(std::integral_constant<int, 16>{}); break;
+    default: go(std::integral_constant<int, 0>{}); break;
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T>
 static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_gv, const void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:71-79
   const bool useN = (trans == OGS_NOTRANS);
@@ -2111,6 +2189,9 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const bool useN = (trans == OGS_TRANS);
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  // a dense list (at least half as many entries as columns): by columns
+  const int32_t *colRow = useN ? p->colRowN : p->colRowT;
+  if (colRow) return launch_scatter_cols<T>(p->Ncols, K, colRow, static_cast<const T *>(d_gv), static_cast<T *>(d_v), st);
   const bool partN = useN && !p->aliased;
   // K = 1 from a T list without empty rows: rows staged in LDS, each entry's
   // row found by counting heads, on the merge-path blocks (measured faster
@@ -2286,19 +2367,21 @@ static int layer_distance(const Plan *p, const Partition &pt, hipStream_t st, in
 }
 
 // Tiled block orders of the K = 1 kernels of a T list without empty rows:
-// for the GatherScatter and Gather over partT[0] and the Scatter over
-// partS[0], each partition's layer distance, its tiled order, and per kernel
-// whether that order measures faster on scratch values (fp64).  Plans whose
-// widest partition has fewer than 16 blocks per CU keep block id order.
+// for the GatherScatter and Gather over partT[0] and the row-wise Scatter
+// over partS[0] (when the list is too sparse to scatter by columns), each
+// partition's layer distance, its tiled order, and per kernel whether that
+// order measures faster on scratch values (fp64).  Plans whose widest
+// partition has fewer than 16 blocks per CU keep block id order.
 static int tile_schedule(Plan *p, hipStream_t st) {
   Partition &pt = p->partT[0];
   Partition &ps = p->partS[0];
-  if (p->nparts == 0 || pt.nblocks < 16 * p->cus || p->Ncols < 2 || p->nnzT == 0 || pt.maxT > kCap ||
-      ps.maxT > kCap || ps.d == pt.d)
-    return 0;
+  if (p->nparts == 0 || pt.nblocks < 16 * p->cus || p->Ncols < 2 || p->nnzT == 0 || pt.maxT > kCap) return 0;
+  // the row-wise scatter's own merge-path blocks (a T list too sparse to
+  // scatter by columns)
+  const bool rowScatter = ps.d != pt.d && ps.maxT <= kCap;
   int DT = 0, DS = 0;
   int rc = layer_distance(p, pt, st, &DT);
-  if (!rc) rc = layer_distance(p, ps, st, &DS);
+  if (!rc && rowScatter) rc = layer_distance(p, ps, st, &DS);
   if (rc || (!DT && !DS)) return rc;
   double *vals = nullptr, *rows = nullptr;
   int *oT = nullptr, *oS = nullptr;
@@ -2361,6 +2444,18 @@ static int tile_schedule(Plan *p, hipStream_t st) {
     ps.tiled = kTiledScatter;
   } else if (oS) {
     (void)hipFree(oS);
+  }
+  return rc;
+}
+
+// colRow (allocated here) of the list (rs, ci) over rows [0, nrows).
+static int build_col_rows(const int32_t *rs, int32_t nrows, const int32_t *ci, int32_t ncols, hipStream_t st,
+                          int32_t **colRow) {
+  int rc = (int)hipMalloc(colRow, sizeof(int32_t) * (size_t)ncols);
+  if (!rc) rc = (int)hipMemsetAsync(*colRow, 0xff, sizeof(int32_t) * (size_t)ncols, st);
+  if (!rc && nrows > 0) {
+    hipLaunchKernelGGL(k_col_rows, dim3(grid_for(nrows, 16384)), dim3(kBlock), 0, st, rs, nrows, ci, *colRow);
+    rc = (int)hipGetLastError();
   }
   return rc;
 }
@@ -2429,6 +2524,8 @@ static void free_plan(Plan *p) {
   if (nn) free_pack(p->packNN);
   if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
+  if (p->colRowN && p->colRowN != p->colRowT) (void)hipFree(p->colRowN);
+  if (p->colRowT) (void)hipFree(p->colRowT);
   if (p->partT[0].order) (void)hipFree(p->partT[0].order);
   if (p->partS[0].order && p->partS[0].order != p->partT[0].order) (void)hipFree(p->partS[0].order);
   for (int j = 0; j < p->nparts; ++j) {
@@ -2510,11 +2607,20 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     else rc = scan_csr(p->rsN, NrowsT, p->ciN, p->nnzN, Ncols, scratch, st, &p->maxRowN, &fl2);
   }
   if (!rc && ((flags | fl2) & 3)) rc = OGS_E_CSR;
+  // column -> row maps of the dense lists, for the by-column scatter
+  if (!rc && Ncols > 0 && 2 * (int64_t)p->nnzT >= Ncols)
+    rc = build_col_rows(p->rsT, NrowsT, p->ciT, Ncols, st, &p->colRowT);
+  if (!rc) {
+    if (p->aliased) p->colRowN = p->colRowT;
+    else if (Ncols > 0 && 2 * (int64_t)p->nnzN >= Ncols)
+      rc = build_col_rows(p->rsN, NrowsT, p->ciN, Ncols, st, &p->colRowN);
+  }
   // T lists without empty rows (every operator that Setup builds) are cut on
   // entries alone: measured faster than merge-path blocks of the same width
   // for GatherScatter and Gather (more entries per workgroup, fewer blocks).
-  // The scatter kernels keep merge-path partitions (partS), measured faster
-  // for them.  N lists may have empty rows and are cut by the merge path.
+  // The row-wise scatter kernels (lists too sparse to scatter by columns)
+  // keep merge-path partitions (partS), measured faster for them.  N lists
+  // may have empty rows and are cut by the merge path.
   const bool cutTOnEntries = (flags & 4) == 0;
   const int lmax = p->maxRowT > p->maxRowN ? p->maxRowT : p->maxRowN;
   if (!rc && lmax > kCap) rc = OGS_E_ROW_TOO_LONG;
@@ -2528,7 +2634,7 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
       else rc = build_partition(p, true, true, NrowsN, W, scratch, st, &p->partN[j]);
     }
     if (!rc) {
-      if (cutTOnEntries) rc = build_partition(p, false, true, NrowsT, W, scratch, st, &p->partS[j]);
+      if (cutTOnEntries && !p->colRowT) rc = build_partition(p, false, true, NrowsT, W, scratch, st, &p->partS[j]);
       else p->partS[j] = p->partT[j];
     }
     p->nparts = j + 1;
@@ -2545,7 +2651,8 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
       if (!rc) rc = build_pack(p->partN[0], NrowsN, p->rsN, p->ciN, true, p->nnzN, scratch, st, &p->packNN);
     }
     if (!rc) {
-      if (cutTOnEntries) rc = build_pack(p->partS[0], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packS);
+      if (p->partS[0].d != p->partT[0].d)
+        rc = build_pack(p->partS[0], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packS);
       else p->packS = p->packT[0];
     }
   }
@@ -2576,6 +2683,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->plainK1 = p->packT[0].plain;
   info->Ncols = p->Ncols;
   info->tileD = p->nparts ? p->partT[0].tileD : 0;
+  info->colsT = p->colRowT != nullptr;
+  info->colsN = p->colRowN != nullptr;
   return OGS_SUCCESS;
 }
 

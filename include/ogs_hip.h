@@ -6,7 +6,7 @@
  *  1. Operator layer (ogs_hip_*): the device half of ogsOperator_t.  A plan
  *     wraps one caller-owned CSR pair (N and T lists) on the device and owns
  *     only what it derives from them (row partitions, column-sorted packs of
- *     the lists).  It replaces the OCCA kernel tables and device overloads
+ *     the lists, each column's row).  It replaces the OCCA kernel tables and device overloads
  *     of the reference:
  *       ogsOperator_t::Gather  (deviceMemory)  libs/ogs/ogsOperator.cpp:159-187
  *       ogsOperator_t::Scatter (deviceMemory)  libs/ogs/ogsOperator.cpp:266-293
@@ -115,7 +115,11 @@ int ogs_hip_gather(ogs_hip_plan_t plan, int type, int op, int trans, int32_t K,
                    void *d_gv, const void *d_v, void *stream);
 /* ogsOperator_t::Scatter (ogsOperator.cpp:206-238, device 266-293):
  * d_v[k + col*K] = d_gv[k + row*K].  Trans: rows [0,NrowsN) over the N lists;
- * NoTrans/Sym: rows [0,NrowsT) over the T lists. */
+ * NoTrans/Sym: rows [0,NrowsT) over the T lists.  A column held by several
+ * rows (never so for Setup-built lists; the reference's row-parallel loops
+ * leave any one of their values) gets the largest such row's value where the
+ * list scatters by columns (colsT / colsN of ogs_hip_plan_info), any one of
+ * them where it scatters by rows. */
 int ogs_hip_scatter(ogs_hip_plan_t plan, int type, int trans, int32_t K,
                     void *d_v, const void *d_gv, void *stream);
 /* OKL extract (ogsKernels.okl:167-177; host ogsOperator.cpp:540-559):
@@ -129,10 +133,13 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * (packedK1) or in CSR order (plainK1), and the layer distance (in blocks)
  * that the K=1 GatherScatter / Gather block order is tiled for (tileD; 0 =
  * block id order).  Plan creation picks a tiled order, per kernel, only where
- * it measures faster on scratch values. */
+ * it measures faster on scratch values.  colsT / colsN: 1 where Scatter over
+ * that list runs by columns (a list with at least Ncols/2 entries: the plan
+ * holds each column's row), 0 where it runs by rows. */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
+  int32_t colsT, colsN;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
 

@@ -4,8 +4,9 @@ the pinned CPU oracle, bit-exact, on the same seeded inputs.
 Covers every {type x op x trans x K} of GatherScatter / Gather / Scatter on
 Unsigned, Signed (ragged, flagged) and Signed-unique setups, the reference's
 own "-cc" check (gslib, Error = 0), the 24^3 p=3 configuration, the full
-48^3 p=7 headline size, extract, hand-built operators with long rows, and
-the error contract.
+48^3 p=7 headline size, extract, hand-built operators with long rows,
+Scatter by columns (dense lists) and by rows (sparse lists), and the error
+contract.
 """
 import numpy as np
 import pytest
@@ -452,6 +453,80 @@ def test_hand_built_operator_long_rows(torch, D):
             d = to_dev(torch, v)
             plan.Scatter(d, to_dev(torch, gvals), K, trans)
             assert same(to_host(torch, d), ws), (K, trans)
+
+
+def rand_csr(rng, Ncols, nnz, lens_hi=9):
+    """Rows of 1..lens_hi-1 distinct columns drawn from [0, Ncols) without
+    repeats, nnz entries in all."""
+    lens = rng.integers(1, lens_hi, nnz)
+    cs = np.cumsum(lens)
+    n = int(np.searchsorted(cs, nnz)) + 1
+    lens = lens[:n]
+    lens[-1] -= cs[n - 1] - nnz  # rows sum to nnz, the last one non-empty
+    cols = rng.choice(Ncols, nnz, replace=False).astype(np.int32)
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(cols[rs[r]:rs[r + 1]]) for r in range(len(lens))]).astype(np.int32)
+    return rs, ci
+
+
+@pytest.mark.parametrize("ty", list(TYPES))
+def test_scatter_by_columns_and_by_rows(torch, D, ty):
+    """Scatter over a dense T list (>= Ncols/2 entries: by columns, each
+    column's row held by the plan) and a sparse N list (by rows), and over a
+    plan whose T list is sparse too, for compile-time and runtime K."""
+    rng = np.random.default_rng(40 + ty)
+    Ncols = 20_000
+    rsT, ciT = rand_csr(rng, Ncols, 16_000)
+    NrowsT = len(rsT) - 1
+    rows = np.repeat(np.arange(NrowsT), np.diff(rsT))
+    keep = rng.random(len(ciT)) < 0.4
+    ciN = ciT[keep]
+    rsN = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=NrowsT))]).astype(np.int32)
+    dense = O.OracleCsr(rsT, ciT, rsN, ciN, NrowsT, Ncols)
+    p1 = D.Plan(Ncols, NrowsT, NrowsT, to_dev(torch, rsN), to_dev(torch, ciN), to_dev(torch, rsT), to_dev(torch, ciT))
+    rsS, ciS = rand_csr(rng, Ncols, 6_000)
+    sparse = O.OracleCsr(rsS, ciS, Ncols=Ncols)
+    dS, cS = to_dev(torch, rsS), to_dev(torch, ciS)
+    p2 = D.Plan(Ncols, len(rsS) - 1, len(rsS) - 1, dS, cS, dS, cS)
+    i1, i2 = p1.info(), p2.info()
+    assert (i1["colsT"], i1["colsN"], i2["colsT"], i2["colsN"]) == (1, 0, 0, 0), (i1, i2)
+    dt = TYPES[ty]
+    for plan, ref, nrows in ((p1, dense, NrowsT), (p2, sparse, len(rsS) - 1)):
+        for K in (1, 2, 3, 5, 16):
+            for trans in TRANS:
+                v = values(rng, dt, Ncols * K)
+                gv = values(rng, dt, nrows * K)
+                want = ref.scatter(v.copy(), gv, K, trans)
+                d = to_dev(torch, v)
+                plan.Scatter(d, to_dev(torch, gv), K, trans)
+                assert same(to_host(torch, d), want), (plan is p1, K, trans)
+
+
+def test_scatter_by_columns_largest_row_wins(torch, D):
+    """A hand-built list whose columns sit in several rows (Setup never
+    builds one).  The reference's host loop runs over rows in parallel, so
+    any of those rows' values may land last; scattering by columns always
+    leaves the largest row's value -- a row-order sequential loop's result,
+    brute force here."""
+    rng = np.random.default_rng(44)
+    Ncols = 5000
+    NrowsT = 3000
+    lens = rng.integers(1, 5, NrowsT)
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(Ncols, l, replace=False)) for l in lens]).astype(np.int32)
+    assert len(np.unique(ci)) < len(ci)  # repeated columns across rows
+    d_rs, d_ci = to_dev(torch, rs), to_dev(torch, ci)
+    plan = D.Plan(Ncols, NrowsT, NrowsT, d_rs, d_ci, d_rs, d_ci)
+    assert plan.info()["colsT"] == 1
+    for K in (1, 3, 7):
+        v = rng.uniform(-1, 1, Ncols * K)
+        gv = rng.uniform(-1, 1, NrowsT * K)
+        want = v.copy().reshape(Ncols, K)
+        for r in range(NrowsT):
+            want[ci[rs[r]:rs[r + 1]]] = gv[r * K:(r + 1) * K]
+        d = to_dev(torch, v)
+        plan.Scatter(d, to_dev(torch, gv), K, O.NOTRANS)
+        assert same(to_host(torch, d), want.ravel()), K
 
 
 def test_partition_cut_follows_empty_rows(torch, D):
