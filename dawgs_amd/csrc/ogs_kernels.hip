@@ -2189,22 +2189,28 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const bool useN = (trans == OGS_TRANS);
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
-  // a dense list (at least half as many entries as columns): by columns
+  // a dense list (at least half as many entries as columns) scatters by
+  // columns where a column's fields take at most 8 bytes: row-wise kernels
+  // write such values in parts of lines that blocks far apart share (fp64
+  // K = 1 at 48^3 p=7: 198 -> 188 us; 77^3 p=4: 256 -> 190 us).  With wider
+  // fields every column re-reads its row's values (fp64 K = 16: 2.20 ms
+  // row-wise -> 2.38 ms by columns), so the row-wise kernels keep those.
   const int32_t *colRow = useN ? p->colRowN : p->colRowT;
-  if (colRow) return launch_scatter_cols<T>(p->Ncols, K, colRow, static_cast<const T *>(d_gv), static_cast<T *>(d_v), st);
+  const T *gv = static_cast<const T *>(d_gv);
+  T *v = static_cast<T *>(d_v);
+  if (colRow && (int64_t)K * (int64_t)sizeof(T) <= 8) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st);
   const bool partN = useN && !p->aliased;
-  // K = 1 from a T list without empty rows: rows staged in LDS, each entry's
-  // row found by counting heads, on the merge-path blocks (measured faster
-  // for this kernel than blocks cut on entries, except at degree 1); on a
-  // small list, on the narrower entry-cut blocks (with packs) that fill the
-  // device
-  if (K == 1 && !partN && !p->partT[0].rowsInKey && p->nnzT > 0 && p->partS[0].maxT <= kCap) {
+  // K = 1 from a sparse T list without empty rows: rows staged in LDS, each
+  // entry's row found by counting heads, on the merge-path blocks (measured
+  // faster for this kernel than blocks cut on entries, except at degree 1);
+  // on a small list, on the narrower entry-cut blocks (with packs) that fill
+  // the device
+  if (K == 1 && !colRow && !partN && !p->partT[0].rowsInKey && p->nnzT > 0 && p->partS[0].maxT <= kCap) {
     const Partition *vp = &p->partS[0];
     if (vp->nblocks < kFillPerCu * p->cus && p->nparts > 1) vp = fill_device(p, &p->partT[1]);
     const ListView S = list_view(p, false, vp, false);
     hipLaunchKernelGGL((k_scatter_vec<T, 1>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT, S.pk,
-                       S.base, S.stride, static_cast<const T *>(d_gv), static_cast<T *>(d_v),
-                       (vp->tiled & kTiledScatter) ? vp->order : nullptr);
+                       S.base, S.stride, gv, v, (vp->tiled & kTiledScatter) ? vp->order : nullptr);
     return (int)hipGetLastError();
   }
   // other field counts with a compile-time shape: the same kernel on the
@@ -2214,10 +2220,10 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
     const Partition *vp = vec_partition<T>(p, K, 0);
     if (vp) {
       vp = fill_device(p, vp);
-      return launch_scatter_vec<T>(K, vp, list_view(p, false, vp, false), p, static_cast<const T *>(d_gv),
-                                   static_cast<T *>(d_v), st);
+      return launch_scatter_vec<T>(K, vp, list_view(p, false, vp, false), p, gv, v, st);
     }
   }
+  if (colRow) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st);
   const Partition *parts = partN ? p->partN : p->partS;
   int kc = 0;
   const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
@@ -2227,11 +2233,9 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const dim3 grid(pt->nblocks), block(kBlock);
   if (K == 1) {
     const ListView S = list_view(p, useN && !p->aliased, pt, partN);
-    hipLaunchKernelGGL((k_scatter_k1<T>), grid, block, 0, st, pt->d, nrows, S, useN ? 1 : 0,
-                       static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+    hipLaunchKernelGGL((k_scatter_k1<T>), grid, block, 0, st, pt->d, nrows, S, useN ? 1 : 0, gv, v);
   } else {
-    hipLaunchKernelGGL((k_scatter<T>), grid, block, 0, st, pt->d, nrows, sS, sI, useN ? 1 : 0, K, kc,
-                       static_cast<const T *>(d_gv), static_cast<T *>(d_v));
+    hipLaunchKernelGGL((k_scatter<T>), grid, block, 0, st, pt->d, nrows, sS, sI, useN ? 1 : 0, K, kc, gv, v);
   }
   return (int)hipGetLastError();
 }

@@ -1,6 +1,8 @@
 """Summaries of rocprofv3 output for the gather-scatter kernels.
 
     python -m dawgs_amd.profiling stats  <dir-with-*_kernel_stats.csv>
+    python -m dawgs_amd.profiling tail <trace dir> --kernel K --last N [--groups G]
+    python -m dawgs_amd.profiling counters <pmc dir> --counter C --kernel K [--groups G] [--last N]
     python -m dawgs_amd.profiling traffic <FETCH_SIZE run dir> <WRITE_SIZE run dir> \
         --kernel k_gs_k1 --copy-kernel copy --copy-bytes B [--json out.json]
 
@@ -47,6 +49,37 @@ def trace_tail_mean(d: str, kernel: str, last: int) -> dict:
     tail = durs[-last:] if last else durs
     return {"dispatches": len(durs), "tail": len(tail), "tail_mean_ns": sum(tail) / max(len(tail), 1),
             "all_mean_ns": sum(durs) / max(len(durs), 1)}
+
+
+def _groups(vals: list, groups: int, last: int) -> list:
+    """`vals` (in dispatch order) cut into `groups` equal runs, the mean of
+    each run's last `last` values (all when 0)."""
+    n = len(vals) // groups
+    if n * groups != len(vals) or n == 0:
+        raise ValueError(f"{len(vals)} dispatches do not split into {groups} equal groups")
+    out = []
+    for g in range(groups):
+        run = vals[g * n:(g + 1) * n]
+        run = run[-last:] if last else run
+        out.append(sum(run) / len(run))
+    return out
+
+
+def trace_group_means(d: str, kernel: str, groups: int, last: int) -> list:
+    """Mean duration (ns) per group of a kernel's dispatches in time order,
+    e.g. one group per degree of a sweep that launches the same count each."""
+    rows = [r for r in _rows(d, "kernel_trace.csv") if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    return _groups([int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows], groups, last)
+
+
+def counter_group_means(d: str, counter: str, kernel: str, groups: int, last: int) -> list:
+    """Mean counter value (KiB) per group of a kernel's dispatches in
+    dispatch order."""
+    v = [(int(r["Dispatch_Id"]), float(r["Counter_Value"])) for r in _rows(d, "counter_collection.csv")
+         if r["Counter_Name"] == counter and kernel in r["Kernel_Name"]]
+    v.sort()
+    return _groups([x for _, x in v], groups, last)
 
 
 def counter_means(d: str, counter: str, last: int = 0):
@@ -96,6 +129,13 @@ def main(argv=None) -> int:
     tl.add_argument("dir")
     tl.add_argument("--kernel", required=True)
     tl.add_argument("--last", type=int, required=True)
+    tl.add_argument("--groups", type=int, default=1, help="split the dispatches into G equal runs")
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--counter", required=True)
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--groups", type=int, default=1)
+    c.add_argument("--last", type=int, default=0)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -108,8 +148,13 @@ def main(argv=None) -> int:
     if a.cmd == "stats":
         for k in sorted(kernel_stats(a.dir), key=lambda r: -r["pct"]):
             print(f"{k['avg_ns'] / 1e3:10.1f} us avg  {k['calls']:5d} calls  {k['pct']:6.2f}%  {k['name'][:110]}")
+    elif a.cmd == "tail" and a.groups > 1:
+        print(json.dumps({"group_tail_mean_ns": trace_group_means(a.dir, a.kernel, a.groups, a.last)}))
     elif a.cmd == "tail":
         print(json.dumps(trace_tail_mean(a.dir, a.kernel, a.last), indent=1))
+    elif a.cmd == "counters":
+        print(json.dumps({"counter": a.counter,
+                          "group_mean_KiB": counter_group_means(a.dir, a.counter, a.kernel, a.groups, a.last)}))
     else:
         r = traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_kernel, a.copy_bytes, a.last)
         print(json.dumps(r, indent=1))

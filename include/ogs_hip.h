@@ -115,11 +115,12 @@ int ogs_hip_gather(ogs_hip_plan_t plan, int type, int op, int trans, int32_t K,
                    void *d_gv, const void *d_v, void *stream);
 /* ogsOperator_t::Scatter (ogsOperator.cpp:206-238, device 266-293):
  * d_v[k + col*K] = d_gv[k + row*K].  Trans: rows [0,NrowsN) over the N lists;
- * NoTrans/Sym: rows [0,NrowsT) over the T lists.  A column held by several
- * rows (never so for Setup-built lists; the reference's row-parallel loops
- * leave any one of their values) gets the largest such row's value where the
- * list scatters by columns (colsT / colsN of ogs_hip_plan_info), any one of
- * them where it scatters by rows. */
+ * NoTrans/Sym: rows [0,NrowsT) over the T lists.  Over a list of at least
+ * Ncols/2 entries (colsT / colsN of ogs_hip_plan_info) it runs by columns
+ * when a column's K values take at most 8 bytes or K is not 2, 3, 4, 8 or
+ * 16, else by rows.  A column held by several rows (never so for Setup-built
+ * lists; the reference's row-parallel loops leave any one of their values)
+ * gets the largest such row's value by columns, any one of them by rows. */
 int ogs_hip_scatter(ogs_hip_plan_t plan, int type, int trans, int32_t K,
                     void *d_v, const void *d_gv, void *stream);
 /* OKL extract (ogsKernels.okl:167-177; host ogsOperator.cpp:540-559):
@@ -133,9 +134,9 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * (packedK1) or in CSR order (plainK1), and the layer distance (in blocks)
  * that the K=1 GatherScatter / Gather block order is tiled for (tileD; 0 =
  * block id order).  Plan creation picks a tiled order, per kernel, only where
- * it measures faster on scratch values.  colsT / colsN: 1 where Scatter over
- * that list runs by columns (a list with at least Ncols/2 entries: the plan
- * holds each column's row), 0 where it runs by rows. */
+ * it measures faster on scratch values.  colsT / colsN: 1 where the plan
+ * holds each column's row for that list (a list with at least Ncols/2
+ * entries), so that Scatter can run by columns (see ogs_hip_scatter). */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
