@@ -2452,11 +2452,16 @@ static int tile_schedule(Plan *p, hipStream_t st) {
   return rc;
 }
 
-// colRow (allocated here) of the list (rs, ci) over rows [0, nrows).
+// colRow (allocated here) of the list (rs, ci) over rows [0, nrows); left
+// null, and the list scattered by rows, when the device has no room for it.
 static int build_col_rows(const int32_t *rs, int32_t nrows, const int32_t *ci, int32_t ncols, hipStream_t st,
                           int32_t **colRow) {
-  int rc = (int)hipMalloc(colRow, sizeof(int32_t) * (size_t)ncols);
-  if (!rc) rc = (int)hipMemsetAsync(*colRow, 0xff, sizeof(int32_t) * (size_t)ncols, st);
+  if (hipMalloc(colRow, sizeof(int32_t) * (size_t)ncols) != hipSuccess) {
+    (void)hipGetLastError();
+    *colRow = nullptr;
+    return 0;
+  }
+  int rc = (int)hipMemsetAsync(*colRow, 0xff, sizeof(int32_t) * (size_t)ncols, st);
   if (!rc && nrows > 0) {
     hipLaunchKernelGGL(k_col_rows, dim3(grid_for(nrows, 16384)), dim3(kBlock), 0, st, rs, nrows, ci, *colRow);
     rc = (int)hipGetLastError();
