@@ -91,6 +91,11 @@ struct ogs_hip_plan_s {
   // colRowT for an Unsigned operator
   int32_t *colRowT = nullptr;
   int32_t *colRowN = nullptr;
+  // per column, the N-list member of its T row (-1: no row; -2: a row with
+  // no N member), for operators whose N list holds at most one member per
+  // row and whose T list has colRowT (else null): GatherScatter NoTrans K = 1
+  // by columns
+  int32_t *colOwner = nullptr;
 };
 
 namespace ogs_hip {

@@ -1295,6 +1295,52 @@ __global__ __launch_bounds__(kBlock) void k_col_rows(const int32_t *__restrict__
     for (int j = rs[r]; j < rs[r + 1]; ++j) atomicMax(colRow + ci[j], (int32_t)r);
 }
 
+// owner[c] for the by-column GatherScatter NoTrans: the N-list member of the
+// T row holding column c (-1: no row; -2: a row with no N member), for an N
+// list of at most one member per row.
+__global__ __launch_bounds__(kBlock) void k_col_owner(int32_t ncols, const int32_t *__restrict__ colRow,
+                                                      const int32_t *__restrict__ rsN,
+                                                      const int32_t *__restrict__ ciN, int32_t *__restrict__ owner) {
+  for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < ncols; c += (int64_t)gridDim.x * kBlock) {
+    const int r = colRow[c];
+    owner[c] = r < 0 ? -1 : (rsN[r + 1] > rsN[r] ? ciN[rsN[r]] : -2);
+  }
+}
+
+// GatherScatter NoTrans, K = 1, of an operator whose N list holds at most
+// one member per row (Signed unique): by columns, every column with a row
+// takes f(v[owner]) = fin(apply(init, v[owner])) -- the row's reduction over
+// its single N member in the host's arithmetic -- or fin(init) for a row
+// without one.  The owner's own lane may rewrite v[owner] meanwhile, but
+// f(f(x)) = f(x) for every op and type (x + 0, x * 1, compare-select with the
+// identity, and the storage rounding are idempotent), so a column reads the
+// same result either way.  Workgroup b covers columns [b*kColVals,
+// (b+1)*kColVals), stored in order.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gs_owner(int32_t ncols, const int32_t *__restrict__ owner, T *v) {
+  using R = LRed<T, OP>;
+  constexpr int U = kColVals / kBlock;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c0 = (int64_t)b * kColVals + threadIdx.x;
+  int o[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t c = c0 + u * kBlock;
+    o[u] = c < ncols ? __builtin_nontemporal_load(owner + c) : -1;
+  }
+  T x[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (o[u] >= 0) x[u] = v[o[u]];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (o[u] == -1) continue;
+    typename R::A a = R::init();
+    if (o[u] >= 0) R::apply(a, Lds<T>::in(x[u]));
+    v[c0 + u * kBlock] = Lds<T>::out(R::fin(a));
+  }
+}
+
 // GatherScatter with two lists (NoTrans / Trans of an operator whose N and T
 // lists differ), KT interleaved fields (compile-time shape), over an
 // entry-cut T partition whose blocks hold at most CAP/2 entries of either
@@ -1986,6 +2032,26 @@ static int launch_gs2(int K, const Partition *vp, const ListView &G, int gN, con
 }
 
 template <typename T>
+static int launch_gs_owner(int op, const Plan *p, T *v, hipStream_t st) {
+  if (p->Ncols == 0) return OGS_SUCCESS;
+  const dim3 grid((unsigned)(((int64_t)p->Ncols + kColVals - 1) / kColVals)), block(kBlock);
+  switch (op) {
+    case OGS_ADD: hipLaunchKernelGGL((k_gs_owner<T, OGS_ADD>), grid, block, 0, st, p->Ncols, p->colOwner, v); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) {
+        hipLaunchKernelGGL((k_gs_owner<T, OGS_MUL>), grid, block, 0, st, p->Ncols, p->colOwner, v);
+        break;
+      } else {
+        return OGS_E_NOT_SUPPORTED;
+      }
+    case OGS_MAX: hipLaunchKernelGGL((k_gs_owner<T, OGS_MAX>), grid, block, 0, st, p->Ncols, p->colOwner, v); break;
+    case OGS_MIN: hipLaunchKernelGGL((k_gs_owner<T, OGS_MIN>), grid, block, 0, st, p->Ncols, p->colOwner, v); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T>
 static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:321-339
   int nrows;
@@ -1998,6 +2064,8 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
     nrows = p->NrowsT; gListN = true; sListN = false;
   }
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
+  // NoTrans K = 1 with at most one N member per row: by columns
+  if (trans == OGS_NOTRANS && K == 1 && p->colOwner) return launch_gs_owner<T>(op, p, static_cast<T *>(d_v), st);
   const int32_t *gS = gListN ? p->rsN : p->rsT, *gI = gListN ? p->ciN : p->ciT;
   const int32_t *sS = sListN ? p->rsN : p->rsT, *sI = sListN ? p->ciN : p->ciT;
   const int gN = gListN ? 1 : 0, sN = sListN ? 1 : 0;
@@ -2533,6 +2601,7 @@ static void free_plan(Plan *p) {
   if (nn) free_pack(p->packNN);
   if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
+  if (p->colOwner) (void)hipFree(p->colOwner);
   if (p->colRowN && p->colRowN != p->colRowT) (void)hipFree(p->colRowN);
   if (p->colRowT) (void)hipFree(p->colRowT);
   if (p->partT[0].order) (void)hipFree(p->partT[0].order);
@@ -2623,6 +2692,16 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     if (p->aliased) p->colRowN = p->colRowT;
     else if (Ncols > 0 && 2 * (int64_t)p->nnzN >= Ncols)
       rc = build_col_rows(p->rsN, NrowsT, p->ciN, Ncols, st, &p->colRowN);
+  }
+  if (!rc && !p->aliased && p->colRowT && p->maxRowN <= 1) {
+    if (hipMalloc(&p->colOwner, sizeof(int32_t) * (size_t)Ncols) != hipSuccess) {
+      (void)hipGetLastError();  // no room: the row-based kernels run this GatherScatter
+      p->colOwner = nullptr;
+    } else {
+      hipLaunchKernelGGL(k_col_owner, dim3(grid_for(Ncols, 16384)), dim3(kBlock), 0, st, Ncols, p->colRowT, p->rsN,
+                         p->ciN, p->colOwner);
+      rc = (int)hipGetLastError();
+    }
   }
   // T lists without empty rows (every operator that Setup builds) are cut on
   // entries alone: measured faster than merge-path blocks of the same width

@@ -372,6 +372,33 @@ def test_special_values(torch, D):
             assert same(to_host(torch, d), want), (np.dtype(dt).name, op)
 
 
+def test_single_owner_notrans_special_values(torch, D):
+    """GatherScatter NoTrans K = 1 on Signed operators whose N list holds at
+    most one member per row (the by-column path: each column takes f of its
+    row's owner value, f = the op from its identity): -0.0 becomes +0.0 under
+    Add, NaN and infinities follow the host's compare-select, and a group
+    with no positive id gets the identity -- on every column, whichever
+    order the columns are written in."""
+    ids = np.array([1, -1, -1, -2, -2, 3, -3, 4, -4, -4, -4, 5, 0, -6, 6, -7], np.int64)
+    inf, nan = np.inf, np.nan
+    fv = np.array([-0.0, 1.0, 2.0, 5.0, 6.0, nan, 3.0, inf, 0.0, 1.0, 2.0, -inf, 9.0, 4.0, -0.0, 8.0])
+    for unique in (False, True):
+        o = O.OracleOgs(ids.copy(), O.SIGNED, unique)
+        g = D.ogs_t()
+        dids = ids.copy()
+        g.Setup(len(dids), dids, D.Signed, D.Auto, unique, False)
+        info = g.plan.info()
+        assert info["maxRowN"] <= 1 and info["colsT"] == 1, info
+        for dt in (np.float64, np.float32, np.int32, np.int64):
+            x = fv.astype(dt) if np.issubdtype(dt, np.floating) else \
+                np.array([7, -3, 2, np.iinfo(dt).max, 5, -np.iinfo(dt).max, 1, 0, 4, 4, 9, -8, 6, 2, 3, 1], dt)
+            for op in OPS:
+                want = o.op_gather_scatter(x.copy(), 1, op, O.NOTRANS)
+                d = to_dev(torch, x)
+                g.plan.GatherScatter(d, 1, op, O.NOTRANS)
+                assert same(to_host(torch, d), want), (unique, np.dtype(dt).name, op)
+
+
 def test_row_limits_and_errors(torch, D):
     from dawgs_amd._lib import OgsError
     g = D.ogs_t()
