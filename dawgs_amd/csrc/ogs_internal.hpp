@@ -29,6 +29,10 @@ constexpr uint32_t kPackOffMask = (1u << kPackOffBits) - 1;
 static_assert(kPackSegShift == 30, "pack word is 2 + 18 + 1 + 11 bits");
 constexpr int kPackSegs = 4;
 constexpr int kPackParts = 5;  // T-list packs for partitions 0..4 (K up to 16 in one pass)
+// Entries per block of the three-field partitions: a block stages 16 KiB of
+// values in LDS, three per entry, of 4 bytes (float, int, compensated float,
+// bfloat16 widened to float) or 8 bytes (double, long long).
+constexpr int kCap3[2] = {16384 / (4 * 3), 16384 / (8 * 3)};
 static_assert(kCap <= (1 << kPackPosBits), "CSR positions of a block must fit the pack word");
 
 // One row partition: nblocks+1 entries {row, rowStartsT[row],
@@ -86,6 +90,12 @@ struct ogs_hip_plan_s {
   ogs_hip::Pack packNT;
   ogs_hip::Pack packNN;
   ogs_hip::Pack packS;  // the T list over partS[0] (a copy of packT[0] when partS is partT)
+  // T lists cut on entries into blocks of at most kCap3[i] entries, with
+  // their packs: the one-list GatherScatter with three fields runs whole
+  // blocks of its shape instead of the next power-of-two width below it
+  // (nblocks 0: not built, a T list with empty rows)
+  ogs_hip::Partition part3[2];
+  ogs_hip::Pack pack3[2];
   // per column, the largest row of the list holding it (-1: none), for lists
   // with at least half as many entries as columns (else null); colRowN is
   // colRowT for an Unsigned operator

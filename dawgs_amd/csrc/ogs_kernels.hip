@@ -1897,6 +1897,8 @@ static ListView list_view(const Plan *p, bool listN, const Partition *pt, bool p
   const Pack *pk = nullptr;
   if (pt == &p->partS[0]) {
     if (!listN) pk = &p->packS;
+  } else if (pt == &p->part3[0] || pt == &p->part3[1]) {
+    if (!listN || p->aliased) pk = &p->pack3[pt - p->part3];
   } else if (partIsN) {
     if (pt == &p->partN[0] && listN) pk = &p->packNN;
   } else {
@@ -1955,6 +1957,21 @@ static const Partition *vec_partition(const Plan *p, int K, int narrow) {
     }
   }
   return nullptr;
+}
+
+static_assert(VecShape<float, 3>::CAP == kCap3[0] && VecShape<int32_t, 3>::CAP == kCap3[0] &&
+                  VecShape<f32c, 3>::CAP == kCap3[0] && VecShape<bf16s, 3>::CAP == kCap3[0] &&
+                  VecShape<double, 3>::CAP == kCap3[1] && VecShape<int64_t, 3>::CAP == kCap3[1],
+              "three-field partitions match the three-field shapes");
+
+// The three-field partition of the shape's capacity, when K = 3 and the plan
+// built one (a T list cut on entries).
+template <typename T>
+static const Partition *vec_partition3(const Plan *p, int K) {
+  if (K != 3) return nullptr;
+  const int cap = vec_cap_entries<T>(3);
+  const Partition *p3 = &p->part3[cap == kCap3[0] ? 0 : 1];
+  return (p3->nblocks > 0 && p3->maxT <= cap) ? p3 : nullptr;
 }
 
 template <typename T, int OP, int KT>
@@ -2083,8 +2100,12 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   // device on small lists.
   if (sym && p->nnzT == 0) return OGS_SUCCESS;  // no entries: nothing to write
   if (sym && vec_k(K)) {
-    const Partition *vp = vec_partition<T>(p, K, 0);
-    if (vp) vp = fill_device(p, vp);
+    // three fields: whole blocks of the shape when they fill the device
+    const Partition *vp = vec_partition3<T>(p, K);
+    if (!vp || vp->nblocks < kFillPerCu * p->cus) {
+      vp = vec_partition<T>(p, K, 0);
+      if (vp) vp = fill_device(p, vp);
+    }
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
   // two lists, K > 1 with a compile-time shape, T without empty rows: the
@@ -2601,6 +2622,10 @@ static void free_plan(Plan *p) {
   if (nn) free_pack(p->packNN);
   if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
+  for (int i = 0; i < 2; ++i) {
+    free_pack(p->pack3[i]);
+    if (p->part3[i].d) (void)hipFree(p->part3[i].d);
+  }
   if (p->colOwner) (void)hipFree(p->colOwner);
   if (p->colRowN && p->colRowN != p->colRowT) (void)hipFree(p->colRowN);
   if (p->colRowT) (void)hipFree(p->colRowT);
@@ -2743,6 +2768,13 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
         rc = build_pack(p->partS[0], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packS);
       else p->packS = p->packT[0];
     }
+    // three-field partitions and packs
+    for (int i = 0; i < 2 && !rc && cutTOnEntries; ++i) {
+      const int W = kCap3[i] - (lmax > 0 ? lmax - 1 : 0);
+      if (W < 1) continue;
+      rc = build_partition(p, false, false, NrowsT, W, scratch, st, &p->part3[i]);
+      if (!rc) rc = build_pack(p->part3[i], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->pack3[i]);
+    }
   }
   (void)hipFree(scratch);
   if (!rc && cutTOnEntries) rc = tile_schedule(p, st);
@@ -2883,7 +2915,10 @@ static int batch_tables(const std::vector<ogs_hip_batch_desc_t> &in, const std::
     const Partition *pt = nullptr;
     if (p->NrowsT > 0 && p->nparts > 0 && p->nnzT > 0) {  // no entries: nothing to reduce or write
       int kc = K;
-      if (pipe_class(K) >= 0) pt = vec_partition<T>(p, K, K > 1 ? c.narrow : 0);
+      if (pipe_class(K) >= 0) {
+        pt = c.narrow == 0 ? vec_partition3<T>(p, K) : nullptr;
+        if (!pt) pt = vec_partition<T>(p, K, K > 1 ? c.narrow : 0);
+      }
       if (!pt) {
         pt = choose(p->partT, p->nparts, K, [](const Partition &x) { return x.maxT; }, &kc);
         if (!pt) return OGS_E_ROW_TOO_LONG;

@@ -249,6 +249,33 @@ def test_config_24cubed_p3(torch, D):
         assert same(to_host(torch, gq), so.gather(v, 1, op, O.TRANS))
 
 
+def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
+    """K = 3 one-list GatherScatter at 32^3 p=3 (2.1M entries): large enough
+    that every type runs the three-field partitions (blocks of up to 1,365
+    entries for 4-byte values, 682 for 8-byte ones) rather than the
+    power-of-two widths below them; bit-exact against the oracle for each
+    type and op, Sym on the Unsigned and Signed-unique setups and NoTrans on
+    the Unsigned one (its N list is its T list)."""
+    ids = O.make_ids(32, 32, 32, 3)
+    N = len(ids)
+    o = O.OracleOgs(ids, O.UNSIGNED)
+    so = O.OracleOgs(ids, O.SIGNED, True)
+    ogs = D.ogs_t()
+    ogs.Setup(N, ids.copy(), D.Unsigned, D.Auto, False, False)
+    sogs = D.ogs_t()
+    sogs.Setup(N, ids.copy(), D.Signed, D.Auto, True, False)
+    rng = np.random.default_rng(323)
+    cases = [(np.float64, O.ADD), (np.float32, O.ADD), (np.float64, O.MAX), (np.float32, O.MIN)] + \
+            [(dt, op) for dt in (np.int32, np.int64) for op in (O.ADD, O.MIN, O.MAX, O.MUL)]
+    for dt, op in cases:
+        v = values(rng, dt, 3 * N)
+        for ref, dev, trans in ((o, ogs, O.SYM), (so, sogs, O.SYM), (o, ogs, O.NOTRANS)):
+            want = ref.op_gather_scatter(v.copy(), 3, op, trans)
+            d = to_dev(torch, v)
+            dev.GatherScatter(d, 3, op, trans)
+            assert same(to_host(torch, d), want), (np.dtype(dt).name, op, trans)
+
+
 def test_headline_48cubed_p7(torch, D):
     """fp64 Add Sym GatherScatter at 48^3 p=7 (56.6M DOFs): bit-exact against
     the oracle's host loop run on the same CSR, and the size-independent
