@@ -254,8 +254,9 @@ def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
     that every type runs the three-field partitions (blocks of up to 1,365
     entries for 4-byte values, 682 for 8-byte ones) rather than the
     power-of-two widths below them; bit-exact against the oracle for each
-    type and op, Sym on the Unsigned and Signed-unique setups and NoTrans on
-    the Unsigned one (its N list is its T list)."""
+    type and op: GatherScatter Sym on the Unsigned and Signed-unique setups
+    and NoTrans on the Unsigned one (its N list is its T list), and the
+    operator's Gather Trans and Scatter NoTrans (T list) on both."""
     ids = O.make_ids(32, 32, 32, 3)
     N = len(ids)
     o = O.OracleOgs(ids, O.UNSIGNED)
@@ -274,6 +275,16 @@ def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
             d = to_dev(torch, v)
             dev.GatherScatter(d, 3, op, trans)
             assert same(to_host(torch, d), want), (np.dtype(dt).name, op, trans)
+        for ref, dev in ((o, ogs), (so, sogs)):
+            want = ref.op_gather(v, 3, op, O.TRANS)
+            gq = torch.zeros(len(want), dtype=to_dev(torch, v[:1]).dtype, device="cuda")
+            dev.plan.Gather(gq, to_dev(torch, v), 3, op, O.TRANS)
+            assert same(to_host(torch, gq), want), (np.dtype(dt).name, op, "gather")
+            base = values(rng, dt, 3 * N)
+            d = to_dev(torch, base)
+            dev.plan.Scatter(d, gq, 3, O.NOTRANS)
+            assert same(to_host(torch, d), ref.op_scatter(base.copy(), want, 3, O.NOTRANS)), \
+                (np.dtype(dt).name, op, "scatter")
 
 
 def test_headline_48cubed_p7(torch, D):
