@@ -2285,15 +2285,16 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const int nrows = useN ? p->NrowsN : p->NrowsT;
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
   // a dense list (at least half as many entries as columns) scatters by
-  // columns where a column's fields take at most 8 bytes: row-wise kernels
+  // columns where a column's fields take at most 16 bytes: row-wise kernels
   // write such values in parts of lines that blocks far apart share (fp64
-  // K = 1 at 48^3 p=7: 198 -> 188 us; 77^3 p=4: 256 -> 190 us).  With wider
-  // fields every column re-reads its row's values (fp64 K = 16: 2.20 ms
+  // K = 1 at 48^3 p=7: 198 -> 188 us; 77^3 p=4: 256 -> 190 us; fp64 K = 2 at
+  // 48^3 p=7: 61.5 -> 68.7 % of HBM peak).  With wider fields every column
+  // re-reads its row's values (fp64 K = 3: 62.4 -> 61.4 %; K = 16: 2.20 ms
   // row-wise -> 2.38 ms by columns), so the row-wise kernels keep those.
   const int32_t *colRow = useN ? p->colRowN : p->colRowT;
   const T *gv = static_cast<const T *>(d_gv);
   T *v = static_cast<T *>(d_v);
-  if (colRow && (int64_t)K * (int64_t)sizeof(T) <= 8) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st);
+  if (colRow && (int64_t)K * (int64_t)sizeof(T) <= 16) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st);
   const bool partN = useN && !p->aliased;
   // K = 1 from a sparse T list without empty rows: rows staged in LDS, each
   // entry's row found by counting heads, on the merge-path blocks (measured

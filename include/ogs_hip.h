@@ -117,7 +117,7 @@ int ogs_hip_gather(ogs_hip_plan_t plan, int type, int op, int trans, int32_t K,
  * d_v[k + col*K] = d_gv[k + row*K].  Trans: rows [0,NrowsN) over the N lists;
  * NoTrans/Sym: rows [0,NrowsT) over the T lists.  Over a list of at least
  * Ncols/2 entries (colsT / colsN of ogs_hip_plan_info) it runs by columns
- * when a column's K values take at most 8 bytes or K is not 2, 3, 4, 8 or
+ * when a column's K values take at most 16 bytes or K is not 2, 3, 4, 8 or
  * 16, else by rows.  A column held by several rows (never so for Setup-built
  * lists; the reference's row-parallel loops leave any one of their values)
  * gets the largest such row's value by columns, any one of them by rows. */

@@ -571,7 +571,7 @@ def test_scatter_by_columns_largest_row_wins(torch, D):
     """A hand-built list whose columns sit in several rows (Setup never
     builds one).  The reference's host loop runs over rows in parallel, so
     any of those rows' values may land last; scattering by columns (K values
-    of at most 8 bytes, or K outside the row-wise kernels' shapes) always
+    of at most 16 bytes, or K outside the row-wise kernels' shapes) always
     leaves the largest row's value -- a row-order sequential loop's result,
     brute force here."""
     rng = np.random.default_rng(44)
@@ -584,7 +584,8 @@ def test_scatter_by_columns_largest_row_wins(torch, D):
     d_rs, d_ci = to_dev(torch, rs), to_dev(torch, ci)
     plan = D.Plan(Ncols, NrowsT, NrowsT, d_rs, d_ci, d_rs, d_ci)
     assert plan.info()["colsT"] == 1
-    for K, dt in ((1, np.float64), (5, np.float64), (7, np.int64), (2, np.float32), (1, np.int32)):
+    for K, dt in ((1, np.float64), (2, np.float64), (5, np.float64), (7, np.int64), (2, np.float32),
+                  (4, np.float32), (1, np.int32)):
         v = values(rng, dt, Ncols * K)
         gv = values(rng, dt, NrowsT * K)
         want = v.copy().reshape(Ncols, K)
