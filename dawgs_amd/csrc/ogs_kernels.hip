@@ -1102,14 +1102,15 @@ __device__ __forceinline__ void stage_heads(const VecStage<SL> &s, const VecMeta
 // found from the head bits, so no row starts are read), each (row, field
 // chunk)'s result stored at its row -- the number of row heads at or before
 // its head's position -- instead of scattered back.
-template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
-                                                       const int32_t *__restrict__ starts,
-                                                       const int32_t *__restrict__ ids,
-                                                       const uint32_t *__restrict__ pk,
-                                                       const int4 *__restrict__ base, int stride,
-                                                       const T *__restrict__ q, T *__restrict__ gq,
-                                                       const int *__restrict__ order) {
+// Gather over a T list through its pack: each row's members reduced in CSR
+// order from the op's identity; the result stored at gq[row] or, with OWN,
+// at the row's only N member: own[c] for the columns c of the row (negative
+// for a row without one, nothing stored), loaded with the values and stored
+// by the lane holding the row's head entry.
+template <typename T, int OP, int KT, bool OWN>
+__device__ __forceinline__ void gather_vec_body(const int4 *part, const int32_t *starts, const int32_t *ids,
+                                                const uint32_t *pk, const int4 *base, int stride, const T *q,
+                                                T *gq, const int *order, const int32_t *own) {
   using V = VecShape<T, KT>;
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
@@ -1134,6 +1135,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
   const int tid = threadIdx.x;
   const int r0 = uniform(m.p0.x);
   const bool packed = s.bb.x >= 0;
+  int ow[OWN ? SL : 1];
+  if constexpr (OWN) {
+#pragma unroll
+    for (int j = 0; j < SL; ++j)
+      ow[j] = tid + j * kBlock < ni ? __builtin_nontemporal_load(own + slot_col(s.w[j], s.bb, packed)) : -1;
+  }
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
     const int it = tid + j * kBlock;
@@ -1163,10 +1170,57 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
     for (int k = 0; k < len; ++k)
 #pragma unroll
       for (int v = 0; v < VW; ++v) R::apply(red[v], sv[k * KT + v]);
-    T *dst = gq + (size_t)(r0 + head_row(hm, pc, i)) * KT + c * VW;
+    if constexpr (OWN) {
+      // the result over the row's head entry, stored below by its slot
 #pragma unroll
-    for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(R::fin(red[v]));
+      for (int v = 0; v < VW; ++v) val[i * KT + c * VW + v] = R::fin(red[v]);
+    } else {
+      T *dst = gq + (size_t)(r0 + head_row(hm, pc, i)) * KT + c * VW;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(R::fin(red[v]));
+    }
   }
+  if constexpr (OWN) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+      const int it = tid + j * kBlock;
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
+      if (it < ni && ow[j] >= 0 && ((hm[ps >> 5] >> (ps & 31)) & 1u)) {
+        const int c = it % C;
+        T *dst = gq + (size_t)ow[j] * KT + c * VW;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(val[ps * KT + c * VW + v]);
+      }
+    }
+  }
+}
+
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
+                                                       const int32_t *__restrict__ starts,
+                                                       const int32_t *__restrict__ ids,
+                                                       const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       const T *__restrict__ q, T *__restrict__ gq,
+                                                       const int *__restrict__ order) {
+  gather_vec_body<T, OP, KT, false>(part, starts, ids, pk, base, stride, q, gq, order, nullptr);
+}
+
+// GatherScatter Trans (rows [0, NrowsN): gather the T lists, scatter the N
+// lists, ogsOperator.cpp:321-339) on an operator whose N list holds at most
+// one member per row: each row's T members reduced as by Gather Trans, the
+// result stored at the row's N member only (the plan's column -> owner map,
+// negative for a row without one; rows from NrowsN on have none).  In place
+// on v: a block reads its rows' members before its barriers and stores after
+// them, and a column sits in one T row (as in every operator Setup builds),
+// so no other block reads what a block stores.
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_trans_owner(const int4 *part, const int32_t *starts,
+                                                           const int32_t *ids, const uint32_t *pk,
+                                                           const int4 *base, int stride, T *v,
+                                                           const int32_t *colOwner) {
+  gather_vec_body<T, OP, KT, true>(part, starts, ids, pk, base, stride, v, v, nullptr, colOwner);
 }
 
 // Scatter of a T list without empty rows, KT interleaved fields: the block's
@@ -2043,6 +2097,40 @@ static const Partition *vec2_partition(const Plan *p, int K) {
 }
 
 template <typename T, int OP>
+static void launch_gs_trans_owner_op(int K, const Partition *vp, const ListView &G, const Plan *p, T *v,
+                                     hipStream_t st) {
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL((k_gs_trans_owner<T, OP, KT>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT,
+                       G.pk, G.base, G.stride, v, p->colOwner);
+  };
+  switch (K) {
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    default: go(std::integral_constant<int, 16>{}); break;
+  }
+}
+
+template <typename T>
+static int launch_gs_trans_owner(int op, int K, const Partition *vp, const ListView &G, const Plan *p, T *v,
+                                 hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_gs_trans_owner_op<T, OGS_ADD>(K, vp, G, p, v, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_gs_trans_owner_op<T, OGS_MUL>(K, vp, G, p, v, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_gs_trans_owner_op<T, OGS_MAX>(K, vp, G, p, v, st); break;
+    case OGS_MIN: launch_gs_trans_owner_op<T, OGS_MIN>(K, vp, G, p, v, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T, int OP>
 static int launch_gs2(int K, const Partition *vp, const ListView &G, int gN, const ListView &S, int sN, T *v,
                       hipStream_t st) {
   auto go = [&](auto kt) {
@@ -2114,6 +2202,13 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   if (sym && vec_k(K)) {
     const Partition *vp = vec_launch_partition<T>(p, K);
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
+  }
+  // Trans with at most one N member per row (Signed unique, with the
+  // column -> owner map): the Gather kernel over the T list, each row's
+  // result stored at its N member
+  if (trans == OGS_TRANS && p->colOwner && vec_k(K) && !p->partT[0].rowsInKey && p->nnzT > 0) {
+    const Partition *vp = vec_launch_partition<T>(p, K);
+    if (vp) return launch_gs_trans_owner<T>(op, K, vp, list_view(p, false, vp, false), p, v, st);
   }
   // two lists, K > 1 with a compile-time shape, T without empty rows: the
   // two-list entry-major kernel (at K = 1 the row-based kernel below, with a
