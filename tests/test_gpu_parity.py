@@ -254,8 +254,9 @@ def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
     that every type runs the three-field partitions (blocks of up to 1,365
     entries for 4-byte values, 682 for 8-byte ones) rather than the
     power-of-two widths below them; bit-exact against the oracle for each
-    type and op: GatherScatter Sym on the Unsigned and Signed-unique setups
-    and NoTrans on the Unsigned one (its N list is its T list), and the
+    type and op: GatherScatter Sym on the Unsigned and Signed-unique setups,
+    NoTrans on the Unsigned one (its N list is its T list) and Trans on the
+    Signed-unique one (rows of the T list stored at their owners), and the
     operator's Gather Trans and Scatter NoTrans (T list) on both."""
     ids = O.make_ids(32, 32, 32, 3)
     N = len(ids)
@@ -270,7 +271,7 @@ def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
             [(dt, op) for dt in (np.int32, np.int64) for op in (O.ADD, O.MIN, O.MAX, O.MUL)]
     for dt, op in cases:
         v = values(rng, dt, 3 * N)
-        for ref, dev, trans in ((o, ogs, O.SYM), (so, sogs, O.SYM), (o, ogs, O.NOTRANS)):
+        for ref, dev, trans in ((o, ogs, O.SYM), (so, sogs, O.SYM), (o, ogs, O.NOTRANS), (so, sogs, O.TRANS)):
             want = ref.op_gather_scatter(v.copy(), 3, op, trans)
             d = to_dev(torch, v)
             dev.GatherScatter(d, 3, op, trans)
