@@ -1370,28 +1370,48 @@ __global__ __launch_bounds__(kBlock) void k_col_owner(int32_t ncols, const int32
 // identity, and the storage rounding are idempotent), so a column reads the
 // same result either way.  Workgroup b covers columns [b*kColVals,
 // (b+1)*kColVals), stored in order.
-template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_gs_owner(int32_t ncols, const int32_t *__restrict__ owner, T *v) {
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_owner(int32_t ncols, int32_t K, int32_t cpb,
+                                                     const int32_t *__restrict__ owner, T *v) {
   using R = LRed<T, OP>;
   constexpr int U = kColVals / kBlock;
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c0 = (int64_t)b * kColVals + threadIdx.x;
-  int o[U];
+  const int64_t c0 = (int64_t)b * cpb;
+  const int64_t left = (int64_t)ncols - c0;
+  const int nc = left < cpb ? (int)left : cpb;
+  const int tid = threadIdx.x;
+  if constexpr (KT > 0) {
+    // item t: field t % KT of column c0 + t / KT
+    const int nv = nc * KT;
+    int o[U], k[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int64_t c = c0 + u * kBlock;
-    o[u] = c < ncols ? __builtin_nontemporal_load(owner + c) : -1;
-  }
-  T x[U];
+    for (int u = 0; u < U; ++u) {
+      const int t = tid + u * kBlock;
+      const int cl = t / KT;
+      k[u] = t - cl * KT;
+      o[u] = t < nv ? __builtin_nontemporal_load(owner + c0 + cl) : -1;
+    }
+    T x[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u)
-    if (o[u] >= 0) x[u] = v[o[u]];
+    for (int u = 0; u < U; ++u)
+      if (o[u] >= 0) x[u] = v[(size_t)o[u] * KT + k[u]];
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if (o[u] == -1) continue;
-    typename R::A a = R::init();
-    if (o[u] >= 0) R::apply(a, Lds<T>::in(x[u]));
-    v[c0 + u * kBlock] = Lds<T>::out(R::fin(a));
+    for (int u = 0; u < U; ++u) {
+      if (o[u] == -1) continue;
+      typename R::A a = R::init();
+      if (o[u] >= 0) R::apply(a, Lds<T>::in(x[u]));
+      v[(size_t)c0 * KT + tid + u * kBlock] = Lds<T>::out(R::fin(a));
+    }
+  } else {
+    const int64_t nv = (int64_t)nc * K;
+    for (int64_t t = tid; t < nv; t += kBlock) {
+      const int64_t cl = t / K, kk = t - cl * K;
+      const int o = owner[c0 + cl];
+      if (o == -1) continue;
+      typename R::A a = R::init();
+      if (o >= 0) R::apply(a, Lds<T>::in(v[(size_t)o * K + kk]));
+      v[(size_t)(c0 + cl) * K + kk] = Lds<T>::out(R::fin(a));
+    }
   }
 }
 
@@ -2148,21 +2168,39 @@ static int launch_gs2(int K, const Partition *vp, const ListView &G, int gN, con
   return (int)hipGetLastError();
 }
 
+// Largest field count of the by-column GatherScatter NoTrans.
+constexpr int kOwnerMaxK = 16;
+
+template <typename T, int OP>
+static void launch_gs_owner_op(int K, const Plan *p, T *v, hipStream_t st) {
+  const int32_t cpb = K <= kColVals ? kColVals / K : 1;
+  const dim3 grid((unsigned)(((int64_t)p->Ncols + cpb - 1) / cpb)), block(kBlock);
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL((k_gs_owner<T, OP, KT>), grid, block, 0, st, p->Ncols, K, cpb, p->colOwner, v);
+  };
+  switch (K) {
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 16: go(std::integral_constant<int, 16>{}); break;
+    default: go(std::integral_constant<int, 0>{}); break;
+  }
+}
+
 template <typename T>
-static int launch_gs_owner(int op, const Plan *p, T *v, hipStream_t st) {
+static int launch_gs_owner(int op, int K, const Plan *p, T *v, hipStream_t st) {
   if (p->Ncols == 0) return OGS_SUCCESS;
-  const dim3 grid((unsigned)(((int64_t)p->Ncols + kColVals - 1) / kColVals)), block(kBlock);
   switch (op) {
-    case OGS_ADD: hipLaunchKernelGGL((k_gs_owner<T, OGS_ADD>), grid, block, 0, st, p->Ncols, p->colOwner, v); break;
+    case OGS_ADD: launch_gs_owner_op<T, OGS_ADD>(K, p, v, st); break;
     case OGS_MUL:
-      if constexpr (kOffered<T, OGS_MUL>) {
-        hipLaunchKernelGGL((k_gs_owner<T, OGS_MUL>), grid, block, 0, st, p->Ncols, p->colOwner, v);
-        break;
-      } else {
-        return OGS_E_NOT_SUPPORTED;
-      }
-    case OGS_MAX: hipLaunchKernelGGL((k_gs_owner<T, OGS_MAX>), grid, block, 0, st, p->Ncols, p->colOwner, v); break;
-    case OGS_MIN: hipLaunchKernelGGL((k_gs_owner<T, OGS_MIN>), grid, block, 0, st, p->Ncols, p->colOwner, v); break;
+      if constexpr (kOffered<T, OGS_MUL>) launch_gs_owner_op<T, OGS_MUL>(K, p, v, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_gs_owner_op<T, OGS_MAX>(K, p, v, st); break;
+    case OGS_MIN: launch_gs_owner_op<T, OGS_MIN>(K, p, v, st); break;
     default: return OGS_E_ARG;
   }
   return (int)hipGetLastError();
@@ -2181,8 +2219,9 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
     nrows = p->NrowsT; gListN = true; sListN = false;
   }
   if (nrows == 0 || p->nparts == 0) return OGS_SUCCESS;
-  // NoTrans K = 1 with at most one N member per row: by columns
-  if (trans == OGS_NOTRANS && K == 1 && p->colOwner) return launch_gs_owner<T>(op, p, static_cast<T *>(d_v), st);
+  // NoTrans with at most one N member per row: by columns
+  if (trans == OGS_NOTRANS && p->colOwner && K <= kOwnerMaxK)
+    return launch_gs_owner<T>(op, K, p, static_cast<T *>(d_v), st);
   const int32_t *gS = gListN ? p->rsN : p->rsT, *gI = gListN ? p->ciN : p->ciT;
   const int32_t *sS = sListN ? p->rsN : p->rsT, *sI = sListN ? p->ciN : p->ciT;
   const int gN = gListN ? 1 : 0, sN = sListN ? 1 : 0;

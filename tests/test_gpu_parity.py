@@ -176,9 +176,11 @@ def test_ogs_level_gather_scatter_unique(torch, D, ty):
 
 
 @pytest.mark.parametrize("K", [8, 16, 100])
-def test_many_fields(torch, K):
-    """K large enough to need field chunks (K=100 runs in passes)."""
-    ids, o, g = mesh("signed")
+@pytest.mark.parametrize("kind", ["signed", "unique"])
+def test_many_fields(torch, K, kind):
+    """K large enough to need field chunks (K=100 runs in passes); on the
+    unique setup NoTrans/Trans run through the column -> owner map."""
+    ids, o, g = mesh(kind)
     rng = np.random.default_rng(K)
     v = rng.uniform(-1, 1, len(ids) * K)
     for trans in TRANS:
@@ -255,8 +257,9 @@ def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
     entries for 4-byte values, 682 for 8-byte ones) rather than the
     power-of-two widths below them; bit-exact against the oracle for each
     type and op: GatherScatter Sym on the Unsigned and Signed-unique setups,
-    NoTrans on the Unsigned one (its N list is its T list) and Trans on the
-    Signed-unique one (rows of the T list stored at their owners), and the
+    NoTrans on the Unsigned one (its N list is its T list), Trans and NoTrans
+    on the Signed-unique one (rows of the T list stored at their owners; each
+    column from its owner), and the
     operator's Gather Trans and Scatter NoTrans (T list) on both."""
     ids = O.make_ids(32, 32, 32, 3)
     N = len(ids)
@@ -271,7 +274,8 @@ def test_three_fields_on_whole_shape_blocks_32cubed_p3(torch, D):
             [(dt, op) for dt in (np.int32, np.int64) for op in (O.ADD, O.MIN, O.MAX, O.MUL)]
     for dt, op in cases:
         v = values(rng, dt, 3 * N)
-        for ref, dev, trans in ((o, ogs, O.SYM), (so, sogs, O.SYM), (o, ogs, O.NOTRANS), (so, sogs, O.TRANS)):
+        for ref, dev, trans in ((o, ogs, O.SYM), (so, sogs, O.SYM), (o, ogs, O.NOTRANS), (so, sogs, O.TRANS),
+                                (so, sogs, O.NOTRANS)):
             want = ref.op_gather_scatter(v.copy(), 3, op, trans)
             d = to_dev(torch, v)
             dev.GatherScatter(d, 3, op, trans)
