@@ -106,6 +106,11 @@ struct ogs_hip_plan_s {
   // row and whose T list has colRowT (else null): GatherScatter NoTrans K = 1
   // by columns
   int32_t *colOwner = nullptr;
+  // one bit per column, set where the column is an N member of its T row,
+  // for operators whose T columns are distinct, whose rows of both lists
+  // ascend and whose N rows are subsets of their T rows (else null):
+  // GatherScatter Trans over the T list alone
+  uint32_t *nMask = nullptr;
 };
 
 namespace ogs_hip {

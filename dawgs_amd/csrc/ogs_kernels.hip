@@ -915,12 +915,15 @@ __device__ __forceinline__ int row_len(const uint32_t *__restrict__ hm, int i) {
   return len + __builtin_ctz(rest);
 }
 
-template <typename T, int OP, int KT, int SL>
+// SCATTER_N false: every entry gathered and scattered (one list).  True:
+// every entry gathered, the results scattered to the entries whose bit in
+// nbits is set (bit j: slot j's entry is an N member) only.
+template <typename T, int OP, int KT, int SL, bool SCATTER_N = false>
 __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta &m,
                                            const int32_t *__restrict__ starts,
                                            const T (&x)[SL][VecShape<T, KT>::VW],
                                            typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
-                                           uint32_t *__restrict__ hm) {
+                                           uint32_t *__restrict__ hm, uint32_t nbits = 0u) {
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
   constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
@@ -1008,7 +1011,7 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
     const int it = tid + j * kBlock;
-    if (it < ni) {
+    if (it < ni && (!SCATTER_N || ((nbits >> j) & 1u))) {
       const int c = it % C;
       const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
       T *d = q + (size_t)slot_col(s.w[j], s.bb, packed) * KT + c * VW;
@@ -1041,6 +1044,44 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_index<T, KT>(s, src, b, m);
   vec_gather<T, KT>(s, x);
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
+}
+
+// GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
+// the T list alone, on a plan with nMask: each entry's N membership from the
+// column mask (a 4-byte word per 32 columns, cache resident), the T pack's
+// blocks and reduction as in the one-list kernel, stores to N members only.
+// Rows from NrowsN on have no N member, so nothing of theirs is stored.
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_trans_mask(const int4 *__restrict__ part,
+                                                          const int32_t *__restrict__ starts,
+                                                          const int32_t *__restrict__ ids,
+                                                          const uint32_t *__restrict__ pk,
+                                                          const int4 *__restrict__ base, int stride,
+                                                          T *__restrict__ q, const uint32_t *__restrict__ nMask) {
+  using V = VecShape<T, KT>;
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
+  const VecSrc src = {part, ids, pk, base, q, 0, stride};
+  VecStage<V::SL> s;
+  VecMeta m;
+  T x[V::SL][V::VW];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  vec_meta(src, b, m);
+  vec_index<T, KT>(s, src, b, m);
+  vec_gather<T, KT>(s, x);
+  const bool packed = s.bb.x >= 0;
+  const int ni = s.n * V::C;
+  uint32_t mw[V::SL];
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j)
+    mw[j] = (int)threadIdx.x + j * kBlock < ni ? nMask[slot_col(s.w[j], s.bb, packed) >> 5] : 0u;
+  uint32_t nbits = 0u;
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) nbits |= ((mw[j] >> (slot_col(s.w[j], s.bb, packed) & 31)) & 1u) << j;
+  vec_finish<T, OP, KT, V::SL, true>(s, m, starts, x, val, head, hm, nbits);
 }
 
 // Exclusive prefix counts of the head-mask words (one wave): pc[w] = heads
@@ -1102,15 +1143,14 @@ __device__ __forceinline__ void stage_heads(const VecStage<SL> &s, const VecMeta
 // found from the head bits, so no row starts are read), each (row, field
 // chunk)'s result stored at its row -- the number of row heads at or before
 // its head's position -- instead of scattered back.
-// Gather over a T list through its pack: each row's members reduced in CSR
-// order from the op's identity; the result stored at gq[row] or, with OWN,
-// at the row's only N member: own[c] for the columns c of the row (negative
-// for a row without one, nothing stored), loaded with the values and stored
-// by the lane holding the row's head entry.
-template <typename T, int OP, int KT, bool OWN>
-__device__ __forceinline__ void gather_vec_body(const int4 *part, const int32_t *starts, const int32_t *ids,
-                                                const uint32_t *pk, const int4 *base, int stride, const T *q,
-                                                T *gq, const int *order, const int32_t *own) {
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
+                                                       const int32_t *__restrict__ starts,
+                                                       const int32_t *__restrict__ ids,
+                                                       const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       const T *__restrict__ q, T *__restrict__ gq,
+                                                       const int *__restrict__ order) {
   using V = VecShape<T, KT>;
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
@@ -1135,12 +1175,6 @@ __device__ __forceinline__ void gather_vec_body(const int4 *part, const int32_t 
   const int tid = threadIdx.x;
   const int r0 = uniform(m.p0.x);
   const bool packed = s.bb.x >= 0;
-  int ow[OWN ? SL : 1];
-  if constexpr (OWN) {
-#pragma unroll
-    for (int j = 0; j < SL; ++j)
-      ow[j] = tid + j * kBlock < ni ? __builtin_nontemporal_load(own + slot_col(s.w[j], s.bb, packed)) : -1;
-  }
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
     const int it = tid + j * kBlock;
@@ -1170,57 +1204,10 @@ __device__ __forceinline__ void gather_vec_body(const int4 *part, const int32_t 
     for (int k = 0; k < len; ++k)
 #pragma unroll
       for (int v = 0; v < VW; ++v) R::apply(red[v], sv[k * KT + v]);
-    if constexpr (OWN) {
-      // the result over the row's head entry, stored below by its slot
+    T *dst = gq + (size_t)(r0 + head_row(hm, pc, i)) * KT + c * VW;
 #pragma unroll
-      for (int v = 0; v < VW; ++v) val[i * KT + c * VW + v] = R::fin(red[v]);
-    } else {
-      T *dst = gq + (size_t)(r0 + head_row(hm, pc, i)) * KT + c * VW;
-#pragma unroll
-      for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(R::fin(red[v]));
-    }
+    for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(R::fin(red[v]));
   }
-  if constexpr (OWN) {
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < SL; ++j) {
-      const int it = tid + j * kBlock;
-      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
-      if (it < ni && ow[j] >= 0 && ((hm[ps >> 5] >> (ps & 31)) & 1u)) {
-        const int c = it % C;
-        T *dst = gq + (size_t)ow[j] * KT + c * VW;
-#pragma unroll
-        for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(val[ps * KT + c * VW + v]);
-      }
-    }
-  }
-}
-
-template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
-                                                       const int32_t *__restrict__ starts,
-                                                       const int32_t *__restrict__ ids,
-                                                       const uint32_t *__restrict__ pk,
-                                                       const int4 *__restrict__ base, int stride,
-                                                       const T *__restrict__ q, T *__restrict__ gq,
-                                                       const int *__restrict__ order) {
-  gather_vec_body<T, OP, KT, false>(part, starts, ids, pk, base, stride, q, gq, order, nullptr);
-}
-
-// GatherScatter Trans (rows [0, NrowsN): gather the T lists, scatter the N
-// lists, ogsOperator.cpp:321-339) on an operator whose N list holds at most
-// one member per row: each row's T members reduced as by Gather Trans, the
-// result stored at the row's N member only (the plan's column -> owner map,
-// negative for a row without one; rows from NrowsN on have none).  In place
-// on v: a block reads its rows' members before its barriers and stores after
-// them, and a column sits in one T row (as in every operator Setup builds),
-// so no other block reads what a block stores.
-template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_gs_trans_owner(const int4 *part, const int32_t *starts,
-                                                           const int32_t *ids, const uint32_t *pk,
-                                                           const int4 *base, int stride, T *v,
-                                                           const int32_t *colOwner) {
-  gather_vec_body<T, OP, KT, true>(part, starts, ids, pk, base, stride, v, v, nullptr, colOwner);
 }
 
 // Scatter of a T list without empty rows, KT interleaved fields: the block's
@@ -1347,6 +1334,39 @@ __global__ __launch_bounds__(kBlock) void k_col_rows(const int32_t *__restrict__
                                                      const int32_t *__restrict__ ci, int32_t *__restrict__ colRow) {
   for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kBlock)
     for (int j = rs[r]; j < rs[r + 1]; ++j) atomicMax(colRow + ci[j], (int32_t)r);
+}
+
+// The N list as one bit per column (nMask), checked against the T list:
+// bad[0] counts T rows that do not strictly ascend, N rows that do not, and
+// N entries outside their T row (colRow[c] != r); with distinct T columns
+// (counted by k_count_cols) a zero count makes the N rows ordered subsets of
+// their T rows.
+__global__ __launch_bounds__(kBlock) void k_n_mask(const int32_t *__restrict__ rsT, const int32_t *__restrict__ ciT,
+                                                   const int32_t *__restrict__ rsN, const int32_t *__restrict__ ciN,
+                                                   int32_t nrows, const int32_t *__restrict__ colRow,
+                                                   uint32_t *__restrict__ mask, int32_t *__restrict__ bad) {
+  int nb = 0;
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kBlock) {
+    for (int j = rsT[r] + 1; j < rsT[r + 1]; ++j) nb += ciT[j] <= ciT[j - 1];
+    for (int j = rsN[r]; j < rsN[r + 1]; ++j) {
+      const int c = ciN[j];
+      if (j > rsN[r] && c <= ciN[j - 1]) ++nb;
+      if (colRow[c] != (int32_t)r) ++nb;
+      else atomicOr(mask + (c >> 5), 1u << (c & 31));
+    }
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
+// Columns held by some row (colRow >= 0), added to cnt.
+__global__ __launch_bounds__(kBlock) void k_count_cols(const int32_t *__restrict__ colRow, int32_t ncols,
+                                                       int32_t *__restrict__ cnt) {
+  int n = 0;
+  for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < ncols; c += (int64_t)gridDim.x * kBlock)
+    n += colRow[c] >= 0;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) n += __shfl_xor(n, d);
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(cnt, n);
 }
 
 // owner[c] for the by-column GatherScatter NoTrans: the N-list member of the
@@ -2117,12 +2137,12 @@ static const Partition *vec2_partition(const Plan *p, int K) {
 }
 
 template <typename T, int OP>
-static void launch_gs_trans_owner_op(int K, const Partition *vp, const ListView &G, const Plan *p, T *v,
-                                     hipStream_t st) {
+static void launch_gs_trans_mask_op(int K, const Partition *vp, const ListView &G, const Plan *p, T *v,
+                                    hipStream_t st) {
   auto go = [&](auto kt) {
     constexpr int KT = decltype(kt)::value;
-    hipLaunchKernelGGL((k_gs_trans_owner<T, OP, KT>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT, p->ciT,
-                       G.pk, G.base, G.stride, v, p->colOwner);
+    hipLaunchKernelGGL((k_gs_trans_mask<T, OP, KT>), dim3(vp->nblocks), dim3(kBlock), 0, st, vp->d, p->rsT,
+                       p->ciT, G.pk, G.base, G.stride, v, p->nMask);
   };
   switch (K) {
     case 1: go(std::integral_constant<int, 1>{}); break;
@@ -2135,16 +2155,16 @@ static void launch_gs_trans_owner_op(int K, const Partition *vp, const ListView 
 }
 
 template <typename T>
-static int launch_gs_trans_owner(int op, int K, const Partition *vp, const ListView &G, const Plan *p, T *v,
-                                 hipStream_t st) {
+static int launch_gs_trans_mask(int op, int K, const Partition *vp, const ListView &G, const Plan *p, T *v,
+                                hipStream_t st) {
   switch (op) {
-    case OGS_ADD: launch_gs_trans_owner_op<T, OGS_ADD>(K, vp, G, p, v, st); break;
+    case OGS_ADD: launch_gs_trans_mask_op<T, OGS_ADD>(K, vp, G, p, v, st); break;
     case OGS_MUL:
-      if constexpr (kOffered<T, OGS_MUL>) launch_gs_trans_owner_op<T, OGS_MUL>(K, vp, G, p, v, st);
+      if constexpr (kOffered<T, OGS_MUL>) launch_gs_trans_mask_op<T, OGS_MUL>(K, vp, G, p, v, st);
       else return OGS_E_NOT_SUPPORTED;
       break;
-    case OGS_MAX: launch_gs_trans_owner_op<T, OGS_MAX>(K, vp, G, p, v, st); break;
-    case OGS_MIN: launch_gs_trans_owner_op<T, OGS_MIN>(K, vp, G, p, v, st); break;
+    case OGS_MAX: launch_gs_trans_mask_op<T, OGS_MAX>(K, vp, G, p, v, st); break;
+    case OGS_MIN: launch_gs_trans_mask_op<T, OGS_MIN>(K, vp, G, p, v, st); break;
     default: return OGS_E_ARG;
   }
   return (int)hipGetLastError();
@@ -2242,12 +2262,14 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
     const Partition *vp = vec_launch_partition<T>(p, K);
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
-  // Trans with at most one N member per row (Signed unique, with the
-  // column -> owner map): the Gather kernel over the T list, each row's
-  // result stored at its N member
-  if (trans == OGS_TRANS && p->colOwner && vec_k(K) && !p->partT[0].rowsInKey && p->nnzT > 0) {
+  // Trans over the T list alone, N membership from the column mask: the
+  // one-list kernel's blocks and traffic, stores to N members only (48^3
+  // p=7 fp64 K = 1: Signed unique 304 -> 236 us, 35 % flagged ids 349 ->
+  // 310 us).  NoTrans stays on the kernels above and below: storing to every
+  // entry of a flagged operator's T blocks measured slower (437 -> 516 us).
+  if (trans == OGS_TRANS && !sym && p->nMask && vec_k(K) && !p->partT[0].rowsInKey && p->nnzT > 0) {
     const Partition *vp = vec_launch_partition<T>(p, K);
-    if (vp) return launch_gs_trans_owner<T>(op, K, vp, list_view(p, false, vp, false), p, v, st);
+    if (vp) return launch_gs_trans_mask<T>(op, K, vp, list_view(p, false, vp, false), p, v, st);
   }
   // two lists, K > 1 with a compile-time shape, T without empty rows: the
   // two-list entry-major kernel (at K = 1 the row-based kernel below, with a
@@ -2701,6 +2723,32 @@ static int build_col_rows(const int32_t *rs, int32_t nrows, const int32_t *ci, i
   return rc;
 }
 
+// nMask of a plan with colRowT (no room or a failed check: none).
+static int build_n_mask(Plan *p, int32_t nrows, int32_t *scratch, hipStream_t st) {
+  const size_t words = ((size_t)p->Ncols + 31) / 32;
+  if (hipMalloc(&p->nMask, sizeof(uint32_t) * words) != hipSuccess) {
+    (void)hipGetLastError();
+    p->nMask = nullptr;
+    return 0;
+  }
+  hipError_t e = hipMemsetAsync(p->nMask, 0, sizeof(uint32_t) * words, st);
+  if (e == hipSuccess) e = hipMemsetAsync(scratch, 0, 2 * sizeof(int32_t), st);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(k_n_mask, dim3(grid_for(nrows, 16384)), dim3(kBlock), 0, st, p->rsT, p->ciT, p->rsN, p->ciN,
+                     nrows, p->colRowT, p->nMask, scratch);
+  hipLaunchKernelGGL(k_count_cols, dim3(grid_for(p->Ncols, 16384)), dim3(kBlock), 0, st, p->colRowT, p->Ncols,
+                     scratch + 1);
+  int32_t h[2] = {0, 0};
+  e = hipMemcpyAsync(h, scratch, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return (int)e;
+  if (h[0] != 0 || h[1] != p->nnzT) {
+    (void)hipFree(p->nMask);
+    p->nMask = nullptr;
+  }
+  return 0;
+}
+
 static int build_partition(Plan *p, bool cutOnN, bool rowsInKey, int nrows, int W, int32_t *scratch, hipStream_t st,
                            Partition *out) {
   const int64_t keys = (int64_t)(cutOnN ? p->nnzN : p->nnzT) + (rowsInKey ? nrows : 0);
@@ -2770,6 +2818,7 @@ static void free_plan(Plan *p) {
     if (p->part3[i].d) (void)hipFree(p->part3[i].d);
   }
   if (p->colOwner) (void)hipFree(p->colOwner);
+  if (p->nMask) (void)hipFree(p->nMask);
   if (p->colRowN && p->colRowN != p->colRowT) (void)hipFree(p->colRowN);
   if (p->colRowT) (void)hipFree(p->colRowT);
   if (p->partT[0].order) (void)hipFree(p->partT[0].order);
@@ -2871,6 +2920,8 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
       rc = (int)hipGetLastError();
     }
   }
+  // the N list as a column mask: GatherScatter Trans over the T list
+  if (!rc && !p->aliased && p->colRowT && p->nnzT > 0) rc = build_n_mask(p, NrowsT, scratch, st);
   // T lists without empty rows (every operator that Setup builds) are cut on
   // entries alone: measured faster than merge-path blocks of the same width
   // for GatherScatter and Gather (more entries per workgroup, fewer blocks).

@@ -539,6 +539,42 @@ def rand_csr(rng, Ncols, nnz, lens_hi=9):
     return rs, ci
 
 
+def test_gather_scatter_trans_by_column_mask_and_without(torch, D):
+    """GatherScatter of hand-built Signed operators whose N rows are ordered
+    subsets of their T rows (the plan keeps the N list as a column mask and
+    runs Trans over the T list alone) and whose N rows are not (an N entry
+    replaced by a column in no T row: the plan keeps no mask and runs the
+    two-list kernels), every trans, K = 1 and 3, against the oracle's host
+    loops."""
+    rng = np.random.default_rng(61)
+    Ncols = 30_000
+    rsT, ciT = rand_csr(rng, Ncols, 24_000)
+    NrowsT = len(rsT) - 1
+    rows = np.repeat(np.arange(NrowsT), np.diff(rsT))
+    keep = rng.random(len(ciT)) < 0.6
+    NrowsN = NrowsT - 7
+    keep &= rows < NrowsN
+    ciN = ciT[keep].copy()
+    rsN = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=NrowsT))]).astype(np.int32)
+    moved = ciN.copy()
+    # one N entry of a multi-member row replaced by a column no T row holds
+    r = int(np.flatnonzero(np.diff(rsN) >= 2)[0])
+    free = np.setdiff1d(np.arange(Ncols), ciT)
+    moved[rsN[r]] = free[0]
+    moved[rsN[r]:rsN[r + 1]] = np.sort(moved[rsN[r]:rsN[r + 1]])
+    for cN in (ciN, moved):
+        ref = O.OracleCsr(rsT, ciT, rsN, cN, NrowsN, Ncols)
+        plan = D.Plan(Ncols, NrowsN, NrowsT, to_dev(torch, rsN), to_dev(torch, cN), to_dev(torch, rsT),
+                      to_dev(torch, ciT))
+        for K in (1, 3):
+            for trans in TRANS:
+                v = rng.uniform(-1, 1, Ncols * K)
+                want = ref.gather_scatter(v.copy(), K, O.ADD, trans)
+                d = to_dev(torch, v)
+                plan.GatherScatter(d, K, O.ADD, trans)
+                assert same(to_host(torch, d), want), (cN is moved, K, trans)
+
+
 @pytest.mark.parametrize("ty", list(TYPES))
 def test_scatter_by_columns_and_by_rows(torch, D, ty):
     """Scatter over a dense T list (>= Ncols/2 entries: by columns, each
