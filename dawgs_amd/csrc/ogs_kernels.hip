@@ -3068,7 +3068,8 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids, const 
 namespace ogs_hip {
 
 struct BatchConfig {
-  int narrow;  // K > 1 descriptors use the next narrower block width
+  int narrow;  // K > 1 descriptors use the next narrower block width (0: K = 3
+               // descriptors on their three-field partition where the plan has one)
   int order;   // 0: items of larger descriptors first, 1: descriptor order
   int fused;   // 1: all field-count classes in one kernel node
 };
