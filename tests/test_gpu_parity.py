@@ -197,6 +197,24 @@ def test_many_fields(torch, K, kind):
         assert same(to_host(torch, s), o.op_scatter(v.copy(), v[: o.NlocalT * K], K, trans))
 
 
+@pytest.mark.parametrize("K", [2, 4, 5, 12])
+@pytest.mark.parametrize("kind", ["signed", "unique"])
+def test_signed_gather_scatter_other_field_counts(torch, kind, K):
+    """GatherScatter NoTrans/Trans of the Signed setups at field counts the
+    other tests skip: compiled shapes (2, 4) and runtime loops (5, 12) of the
+    by-column owner kernel (unique) and of the column-mask Trans kernel and
+    the two-list kernels (both setups)."""
+    ids, o, g = mesh(kind)
+    rng = np.random.default_rng(70 + K)
+    for dt, op in ((np.float64, O.ADD), (np.int32, O.MAX), (np.float32, O.MIN)):
+        v = values(rng, dt, len(ids) * K)
+        for trans in (O.NOTRANS, O.TRANS):
+            want = o.op_gather_scatter(v.copy(), K, op, trans)
+            d = to_dev(torch, v)
+            g.plan.GatherScatter(d, K, op, trans)
+            assert same(to_host(torch, d), want), (np.dtype(dt).name, op, trans)
+
+
 # ------------------------------------------------------------------ reference checks
 @pytest.mark.skipif(not O.ref_available(), reason="gslib comparator not built")
 def test_reference_cc_error_zero(torch, D):
