@@ -447,28 +447,33 @@ def main() -> int:
     t0 = time.perf_counter()
     ids = make_ids(nx, nx, nx, p)
     N = len(ids)
-    ogs = D.ogs_t()
-    ogs.Setup(N, ids, D.Unsigned, D.Auto, False, False)
-    del ids
-    info = ogs.plan.info()
-    log(f"setup {time.perf_counter() - t0:.2f} s: N={N} NgatherGlobal={ogs.NgatherGlobal} "
-        f"nnz={ogs.nnzT} blocks={info['nblocksK1']} W={info['widthK1']} maxRow={info['maxRowT']}")
-
+    # Inputs are made before the Setup, so that the warm-ups follow the
+    # Setup's own device work without an idle gap: on MI355X a memory-bound
+    # kernel that resumes after >= 5 ms of idle runs ~10 calls at full
+    # speed, then 10-30 calls up to 20% slower while the board's power
+    # management reacts, and settles after ~100 calls
+    # (profiles/gs_f64_power_transient_48cubed_p7.md).
     gen = torch.Generator(device="cuda").manual_seed(0)
     q = torch.empty(N, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0, generator=gen)
     stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ogs = D.ogs_t()
+    ogs.Setup(N, ids, D.Unsigned, D.Auto, False, False)
+    del ids
 
     for _ in range(args.warmup):
         ogs.GatherScatter(q, 1, D.Add, D.Sym)
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
     for _ in range(args.steps):
         ogs.GatherScatter(q, 1, D.Add, D.Sym)
     ev1.record(stream)
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / args.steps
+    info = ogs.plan.info()
+    log(f"setup {time.perf_counter() - t0:.2f} s (incl. timing): N={N} NgatherGlobal={ogs.NgatherGlobal} "
+        f"nnz={ogs.nnzT} blocks={info['nblocksK1']} W={info['widthK1']} maxRow={info['maxRowT']}")
 
     B = algorithmic_bytes(ogs.nnzT, ogs.NrowsT, 1, 8)
     achieved = B / (ms * 1e-3) / 1e9

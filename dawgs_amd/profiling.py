@@ -2,6 +2,7 @@
 
     python -m dawgs_amd.profiling stats  <dir-with-*_kernel_stats.csv>
     python -m dawgs_amd.profiling tail <trace dir> --kernel K --last N [--groups G]
+    python -m dawgs_amd.profiling curve <trace dir> --kernel K [--every 10]
     python -m dawgs_amd.profiling counters <pmc dir> --counter C --kernel K [--groups G] [--last N]
     python -m dawgs_amd.profiling traffic <FETCH_SIZE run dir> <WRITE_SIZE run dir> \
         --kernel k_gs_k1 --copy-kernel copy --copy-bytes B [--json out.json]
@@ -49,6 +50,24 @@ def trace_tail_mean(d: str, kernel: str, last: int) -> dict:
     tail = durs[-last:] if last else durs
     return {"dispatches": len(durs), "tail": len(tail), "tail_mean_ns": sum(tail) / max(len(tail), 1),
             "all_mean_ns": sum(durs) / max(len(durs), 1)}
+
+
+def trace_curve(d: str, kernel: str, every: int = 10) -> list:
+    """A kernel's dispatches in time order, cut into runs of `every`: per run
+    the first dispatch's index and start time (ms after the first dispatch),
+    the mean duration (us) and the largest idle gap before a dispatch of the
+    run (us).  Shows how the kernel time moves after setup and idle gaps."""
+    rows = [r for r in _rows(d, "kernel_trace.csv") if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    se = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows]
+    out = []
+    for i in range(0, len(se), every):
+        run = se[i:i + every]
+        gaps = [(s - se[j - 1][1]) / 1e3 for j, (s, _) in enumerate(run, start=i) if j > 0]
+        out.append({"first": i, "t_ms": round((run[0][0] - se[0][0]) / 1e6, 3),
+                    "mean_us": round(sum(e - s for s, e in run) / len(run) / 1e3, 1),
+                    "max_gap_us": round(max(gaps), 1) if gaps else 0.0})
+    return out
 
 
 def _groups(vals: list, groups: int, last: int) -> list:
@@ -130,6 +149,10 @@ def main(argv=None) -> int:
     tl.add_argument("--kernel", required=True)
     tl.add_argument("--last", type=int, required=True)
     tl.add_argument("--groups", type=int, default=1, help="split the dispatches into G equal runs")
+    cv = sub.add_parser("curve")
+    cv.add_argument("dir")
+    cv.add_argument("--kernel", required=True)
+    cv.add_argument("--every", type=int, default=10)
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--counter", required=True)
@@ -152,6 +175,9 @@ def main(argv=None) -> int:
         print(json.dumps({"group_tail_mean_ns": trace_group_means(a.dir, a.kernel, a.groups, a.last)}))
     elif a.cmd == "tail":
         print(json.dumps(trace_tail_mean(a.dir, a.kernel, a.last), indent=1))
+    elif a.cmd == "curve":
+        for r in trace_curve(a.dir, a.kernel, a.every):
+            print(f"{r['first']:5d}  t={r['t_ms']:9.3f} ms  mean {r['mean_us']:7.1f} us  max gap {r['max_gap_us']:9.1f} us")
     elif a.cmd == "counters":
         print(json.dumps({"counter": a.counter,
                           "group_mean_KiB": counter_group_means(a.dir, a.counter, a.kernel, a.groups, a.last)}))
