@@ -19,7 +19,8 @@ _I32P = C.POINTER(C.c_int32)
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
-                 "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN")]
+                 "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN", "deltaE", "deltaFallback")] + \
+               [("bytes", C.c_int64)]
 
 
 class BatchDesc(C.Structure):

@@ -66,6 +66,17 @@ struct Pack {
   int32_t plain = 0;   // blocks left in CSR order
 };
 
+// Delta pack of the T list over partition 0 (K = 1 one-list GatherScatter,
+// k_gs_sym_delta): per block kCap 16-bit words (CSR position | row-head bit
+// | delta code), and kCap / 512 * E 32-bit exception words.
+struct DeltaPack {
+  uint16_t *w = nullptr;               // nblocks * kCap words
+  uint32_t *x = nullptr;               // nblocks * (kCap / 512) * E exception words
+  int32_t E = 0;                       // exception words per 512-entry range
+  uint32_t tbl[4] = {0u, 0u, 0u, 0u};  // byte c: the delta of code c
+  int32_t fallback = 0;                // blocks that run from the 32-bit pack
+};
+
 }  // namespace ogs_hip
 
 // The opaque plan behind ogs_hip_plan_t.
@@ -90,6 +101,7 @@ struct ogs_hip_plan_s {
   ogs_hip::Pack packNT;
   ogs_hip::Pack packNN;
   ogs_hip::Pack packS;  // the T list over partS[0] (a copy of packT[0] when partS is partT)
+  ogs_hip::DeltaPack dpack;  // the T list over partT[0] with 16-bit words (w null: none)
   // T lists cut on entries into blocks of at most kCap3[i] entries, with
   // their packs: the one-list GatherScatter with three fields runs whole
   // blocks of its shape instead of the next power-of-two width below it

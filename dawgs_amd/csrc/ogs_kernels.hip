@@ -915,6 +915,58 @@ __device__ __forceinline__ int row_len(const uint32_t *__restrict__ hm, int i) {
   return len + __builtin_ctz(rest);
 }
 
+// Row reduction of a block whose n entries' values (KT fields each) are
+// staged in LDS at their CSR positions with their row-head bytes (the
+// staging barrier already passed): one lane per (row head, field chunk)
+// folds the row's VW fields in CSR order from the op's identity, members read
+// four at a time, and writes the result back over the row (measured faster
+// than every item summing its own row: rows of several members would be read
+// once per member).  Ends with a barrier.
+template <typename T, int OP, int KT>
+__device__ __forceinline__ void vec_reduce(int n, typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                           uint32_t *__restrict__ hm) {
+  using L = typename Lds<T>::L;
+  using R = LRed<T, OP>;
+  constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
+  const int ni = n * C;
+  const int tid = threadIdx.x;
+  head_mask(head, n, hm);
+  __syncthreads();
+  for (int t = tid; t < ni; t += kBlock) {
+    const int i = t / C, c = t - i * C;
+    if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
+    const int len = row_len(hm, i);
+    typename R::A red[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[v] = R::init();
+    const L *src = val + i * KT + c * VW;
+    int mb = 0;
+    for (; mb + 4 <= len; mb += 4) {
+      L y[4][VW];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) y[u][v] = src[(mb + u) * KT + v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) R::apply(red[v], y[u][v]);
+    }
+    for (; mb < len; ++mb) {
+#pragma unroll
+      for (int v = 0; v < VW; ++v) R::apply(red[v], src[mb * KT + v]);
+    }
+    L res[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) res[v] = R::fin(red[v]);
+    L *dst = val + i * KT + c * VW;
+    for (int r = 0; r < len; ++r)
+#pragma unroll
+      for (int v = 0; v < VW; ++v) dst[r * KT + v] = res[v];
+  }
+  __syncthreads();
+}
+
 // SCATTER_N false: every entry gathered and scattered (one list).  True:
 // every entry gathered, the results scattered to the entries whose bit in
 // nbits is set (bit j: slot j's entry is an N member) only.
@@ -924,8 +976,6 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
                                            const T (&x)[SL][VecShape<T, KT>::VW],
                                            typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
                                            uint32_t *__restrict__ hm, uint32_t nbits = 0u) {
-  using L = typename Lds<T>::L;
-  using R = LRed<T, OP>;
   constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
   const int n = s.n;
   if (n == 0) return;
@@ -968,45 +1018,7 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
     }
   }
   __syncthreads();
-  head_mask(head, n, hm);
-  __syncthreads();
-  // one lane per (row head, field chunk): the row's VW fields reduced in CSR
-  // order, members read four at a time, the result written back over the row
-  // (measured faster than every item summing its own row: rows of several
-  // members would be read once per member)
-  for (int t = tid; t < ni; t += kBlock) {
-    const int i = t / C, c = t - i * C;
-    if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
-    const int len = row_len(hm, i);
-    typename R::A red[VW];
-#pragma unroll
-    for (int v = 0; v < VW; ++v) red[v] = R::init();
-    const L *src = val + i * KT + c * VW;
-    int mb = 0;
-    for (; mb + 4 <= len; mb += 4) {
-      L y[4][VW];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) y[u][v] = src[(mb + u) * KT + v];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) R::apply(red[v], y[u][v]);
-    }
-    for (; mb < len; ++mb) {
-#pragma unroll
-      for (int v = 0; v < VW; ++v) R::apply(red[v], src[mb * KT + v]);
-    }
-    L res[VW];
-#pragma unroll
-    for (int v = 0; v < VW; ++v) res[v] = R::fin(red[v]);
-    L *dst = val + i * KT + c * VW;
-    for (int r = 0; r < len; ++r)
-#pragma unroll
-      for (int v = 0; v < VW; ++v) dst[r * KT + v] = res[v];
-  }
-  __syncthreads();
+  vec_reduce<T, OP, KT>(n, val, head, hm);
   T *q = static_cast<T *>(s.src.q);
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
@@ -1044,6 +1056,149 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_index<T, KT>(s, src, b, m);
   vec_gather<T, KT>(s, x);
   vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
+}
+
+// ---------------------------------------------------------------------------
+// Delta pack: the K = 1 one-list GatherScatter over partition 0 with half the
+// index bytes of the 32-bit pack words.
+//
+// A block's entries in column order (the 32-bit pack's order) form four wave
+// ranges of kDWave = 512 entries: wave w takes entries [512 w, 512 w + 512),
+// slot j of lane l being entry 512 w + 64 j + l (each wave instruction then
+// touches 64 consecutive entries, as in k_gs_sym_vec).  An entry is one
+// 16-bit word: CSR position (11 bits) | row-head bit | delta code (4 bits).
+// Its column is the previous entry's column (in its wave range) plus a delta:
+// code 0 adds 0, codes 1..14 add the plan's table entry (the 14 commonest
+// deltas below 256, one byte each), code 15 adds the wave range's next
+// exception word (32 bits); the first entry of a range is always an
+// exception, holding its column.  The words are stored lane-major within a
+// range (lane l's 8 words contiguous), so one 16-byte load per lane fetches a
+// range's 512 words, and its E exception words come with them (first E
+// lanes, one word each).  A wave recovers its columns with one inclusive scan
+// per slot (DPP).  Blocks that do not fit -- a range with more than E
+// exceptions, or a block the 32-bit pack left in CSR order -- carry
+// kDFallback as every range's first exception word and run from the 32-bit
+// pack.  48^3 p=7: 2.13 index bytes per entry instead of 4.
+constexpr int kDWave = 512;
+constexpr int kDRanges = kCap / kDWave;
+constexpr uint32_t kDEscape = 15u;
+constexpr uint32_t kDFallback = 0xffffffffu;
+constexpr int kDMaxE = 64;
+static_assert(kDRanges * 64 == kBlock && kDWave == 8 * 64, "a wave range is 8 slots of one wave");
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Inclusive prefix sum over the 64 lanes of a wave: Hillis-Steele steps
+// within each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's total into
+// rows 1 and 3 (row_bcast 15) and rows 0-1's total into rows 2 and 3
+// (row_bcast 31).  A lane whose source lies outside its row keeps `old` = 0.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
+  int v = (int)x;
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return (uint32_t)v;
+}
+
+// Lanes below this one whose bit is set in the wave mask m.
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Delta of a code from the table (byte c of tb: code c), by byte permutes
+// (v_perm_b32 picks byte s of {src0:src1}, src1 the low dword; selector 12
+// gives 0).  (A select chain over tb's components was turned by the compiler
+// into a per-lane indexed load of the kernel arguments.)
+__device__ __forceinline__ uint32_t delta_of(uint32_t code, const uint4 &tb) {
+  const uint32_t sel = (code & 7u) | 0x0c0c0c00u;
+  const uint32_t lo = __builtin_amdgcn_perm(tb.y, tb.x, sel);
+  const uint32_t hi = __builtin_amdgcn_perm(tb.w, tb.z, sel);
+  return (code & 8u) ? hi : lo;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict__ part,
+                                                         const int32_t *__restrict__ starts,
+                                                         const int32_t *__restrict__ ids,
+                                                         const uint32_t *__restrict__ pk,
+                                                         const int4 *__restrict__ base, int stride,
+                                                         const u32x4 *__restrict__ dw,
+                                                         const uint32_t *__restrict__ dx, int E, uint4 tb,
+                                                         int ncols, T *__restrict__ q,
+                                                         const int *__restrict__ order) {
+  using V = VecShape<T, 1>;
+  static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int4 p0 = part[b], p1 = part[b + 1];
+  const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
+  const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + wv) * E + lane) : 0u;
+  if ((uint32_t)uniform((int)ex) == kDFallback) {
+    // this block runs from the 32-bit pack (every range carries the mark)
+    const VecSrc src = {part, ids, pk, base, q, 0, stride};
+    VecStage<V::SL> s;
+    VecMeta m;
+    T x[V::SL][1];
+    vec_meta(src, b, m);
+    vec_index<T, 1>(s, src, b, m);
+    vec_gather<T, 1>(s, x);
+    vec_finish<T, OP, 1>(s, m, starts, x, val, head, hm);
+    return;
+  }
+  const int g0 = uniform(p0.y), n = uniform(p1.y) - g0;
+  // columns: per slot, the deltas (exceptions by rank among the range's
+  // escapes, through a lane permute), an inclusive scan, the running carry
+  const uint32_t w2[4] = {wd.x, wd.y, wd.z, wd.w};
+  uint32_t col[V::SL];
+  uint32_t carry = 0u;
+  int run = 0;
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) {
+    const uint32_t code = (w2[j >> 1] >> ((j & 1) * 16 + 12)) & 15u;
+    uint32_t d = delta_of(code, tb);
+    const bool esc = code == kDEscape;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (mk) {
+      const int rank = (run + lanes_below(mk)) & 63;
+      const uint32_t e = (uint32_t)__builtin_amdgcn_ds_bpermute(rank * 4, (int)ex);
+      d = esc ? e : d;
+      run += __builtin_popcountll(mk);
+    }
+    const uint32_t sc = wave_scan_incl(d);
+    const uint32_t c = carry + sc;
+    col[j] = c < (uint32_t)ncols ? c : (uint32_t)(ncols - 1);  // stays in bounds whatever the words hold
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)sc, 63);
+  }
+  T x[V::SL];
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) x[j] = q[col[j]];
+  const int i0 = wv * kDWave + lane;
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) {
+    if (i0 + j * 64 < n) {
+      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
+      const int ps = (int)(wj & kPackPosMask);
+      val[ps] = Lds<T>::in(x[j]);
+      head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
+    }
+  }
+  __syncthreads();
+  vec_reduce<T, OP, 1>(n, val, head, hm);
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) {
+    if (i0 + j * 64 < n) {
+      const int ps = (int)((w2[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
+      q[col[j]] = Lds<T>::out(val[ps]);
+    }
+  }
 }
 
 // GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
@@ -1952,6 +2107,136 @@ __global__ __launch_bounds__(kBlock) void k_count_packed(const int4 *__restrict_
   atomicAdd(out, c);
 }
 
+// Delta-pack builders, one workgroup per block of partition 0, from the
+// block's 32-bit pack words (column order).  Column of sorted entry i:
+__device__ __forceinline__ int pack_col(const uint32_t *__restrict__ pk, const int4 &bb, int stride, int b, int i) {
+  return slot_col(pk[(size_t)b * stride + i], bb, true);
+}
+
+// Code of each delta below 256 under table tb (15: not in the table).
+__device__ __forceinline__ void delta_codes(const uint4 &tb, uint8_t *inv) {
+  const int t = threadIdx.x;  // kBlock == 256 deltas
+  inv[t] = t == 0 ? 0 : (uint8_t)kDEscape;
+  __syncthreads();
+  if (t >= 1 && t < (int)kDEscape) {
+    const uint32_t d = delta_of((uint32_t)t, tb);
+    if (d > 0 && d < 256) inv[d] = (uint8_t)t;
+  }
+  __syncthreads();
+}
+
+// Histogram of the deltas in [1, 256) between consecutive entries of a wave
+// range, over the packed blocks (the candidates of the delta table).
+__global__ __launch_bounds__(kBlock) void k_delta_hist(const int4 *__restrict__ part, const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  h[tid] = 0u;
+  __syncthreads();
+  const int4 bb = base[b];
+  if (bb.x >= 0) {
+    const int n = part[b + 1].y - part[b].y;
+    for (int i = tid; i < n; i += kBlock) {
+      if (i % kDWave == 0) continue;
+      const int d = pack_col(pk, bb, stride, b, i) - pack_col(pk, bb, stride, b, i - 1);
+      if (d > 0 && d < 256) atomicAdd(&h[d], 1u);
+    }
+  }
+  __syncthreads();
+  if (h[tid]) atomicAdd(&hist[tid], h[tid]);
+}
+
+// Exceptions of wave range w of block b under table inv (the first entry of
+// a non-empty range counts): run by wave w, the count in every lane.
+__device__ __forceinline__ int delta_escapes(const uint32_t *__restrict__ pk, const int4 &bb, int stride, int b,
+                                             int n, const uint8_t *inv) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int cnt = 0;
+  for (int j = 0; j < kDWave / 64; ++j) {
+    const int i = w * kDWave + j * 64 + lane;
+    bool esc = false;
+    if (i < n) {
+      if (i % kDWave == 0) {
+        esc = true;
+      } else {
+        const int d = pack_col(pk, bb, stride, b, i) - pack_col(pk, bb, stride, b, i - 1);
+        esc = d < 0 || d >= 256 || inv[d] == kDEscape;
+      }
+    }
+    cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(esc));
+  }
+  return cnt;
+}
+
+// hist[min(m, kDMaxE + 1)] += 1 per packed block, m = the block's most
+// exceptions in one range.
+__global__ __launch_bounds__(kBlock) void k_delta_count(const int4 *__restrict__ part, const uint32_t *__restrict__ pk,
+                                                        const int4 *__restrict__ base, int stride, uint4 tb,
+                                                        uint32_t *__restrict__ hist) {
+  __shared__ uint8_t inv[256];
+  __shared__ int most;
+  const int b = blockIdx.x;
+  delta_codes(tb, inv);
+  const int4 bb = base[b];
+  if (bb.x < 0) return;
+  if (threadIdx.x == 0) most = 0;
+  __syncthreads();
+  const int n = part[b + 1].y - part[b].y;
+  const int c = delta_escapes(pk, bb, stride, b, n, inv);
+  if ((threadIdx.x & 63) == 0) atomicMax(&most, c);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&hist[most < kDMaxE + 1 ? most : kDMaxE + 1], 1u);
+}
+
+// The delta pack's words and exceptions (dw and dx zeroed beforehand); a
+// block that does not fit gets the fallback mark and counts in *fallbacks.
+__global__ __launch_bounds__(kBlock) void k_build_delta(const int4 *__restrict__ part, const uint32_t *__restrict__ pk,
+                                                        const int4 *__restrict__ base, int stride, uint4 tb, int E,
+                                                        uint16_t *__restrict__ dw, uint32_t *__restrict__ dx,
+                                                        int32_t *__restrict__ fallbacks) {
+  __shared__ uint8_t inv[256];
+  __shared__ int over;
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  delta_codes(tb, inv);
+  if (tid == 0) over = 0;
+  __syncthreads();
+  const int4 bb = base[b];
+  const int n = part[b + 1].y - part[b].y;
+  if (bb.x >= 0 && delta_escapes(pk, bb, stride, b, n, inv) > E && lane == 0) over = 1;
+  __syncthreads();
+  uint32_t *xw = dx + ((size_t)b * kDRanges + w) * E;
+  if (bb.x < 0 || over) {
+    if (lane == 0) xw[0] = kDFallback;
+    if (tid == 0) atomicAdd(fallbacks, 1);
+    return;
+  }
+  int run = 0;
+  for (int j = 0; j < kDWave / 64; ++j) {
+    const int i = w * kDWave + j * 64 + lane;
+    uint32_t word = 0u, ev = 0u;
+    bool esc = false;
+    if (i < n) {
+      const uint32_t pw = pk[(size_t)b * stride + i];
+      const int c = slot_col(pw, bb, true);
+      uint32_t code = kDEscape;
+      if (i % kDWave == 0) {
+        ev = (uint32_t)c;
+      } else {
+        const int d = c - pack_col(pk, bb, stride, b, i - 1);
+        if (d >= 0 && d < 256 && inv[d] != kDEscape) code = inv[d];
+        else ev = (uint32_t)d;
+      }
+      esc = code == kDEscape;
+      word = (pw & ((1u << (kPackHeadBit + 1)) - 1u)) | (code << 12);  // position, head bit, code
+    }
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (esc) xw[run + lanes_below(mk)] = ev;
+    run += __builtin_popcountll(mk);
+    dw[(size_t)b * kCap + w * kDWave + lane * (kDWave / 64) + j] = (uint16_t)word;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers.
 
@@ -2226,6 +2511,33 @@ static int launch_gs_owner(int op, int K, const Plan *p, T *v, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+// The K = 1 one-list GatherScatter over partition 0 from the delta pack.
+template <typename T, int OP>
+static void launch_delta_op(const Plan *p, T *v, hipStream_t st) {
+  const Partition &pt = p->partT[0];
+  const Pack &k = p->packT[0];
+  const DeltaPack &d = p->dpack;
+  hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT, k.pk,
+                     k.base, k.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
+                     make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), p->Ncols, v,
+                     (pt.tiled & kTiledGS) ? pt.order : nullptr);
+}
+
+template <typename T>
+static int launch_delta(int op, const Plan *p, T *v, hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_delta_op<T, OGS_ADD>(p, v, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_delta_op<T, OGS_MUL>(p, v, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_delta_op<T, OGS_MAX>(p, v, st); break;
+    case OGS_MIN: launch_delta_op<T, OGS_MIN>(p, v, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
 template <typename T>
 static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:321-339
@@ -2260,6 +2572,7 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   if (sym && p->nnzT == 0) return OGS_SUCCESS;  // no entries: nothing to write
   if (sym && vec_k(K)) {
     const Partition *vp = vec_launch_partition<T>(p, K);
+    if (vp && K == 1 && vp == &p->partT[0] && p->dpack.w) return launch_delta<T>(op, p, v, st);
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
   // Trans over the T list alone, N membership from the column mask: the
@@ -2800,10 +3113,132 @@ static int build_pack(const Partition &pt, int nrows, const int32_t *starts, con
   return OGS_SUCCESS;
 }
 
+static void free_delta(DeltaPack &d) {
+  if (d.w) (void)hipFree(d.w);
+  if (d.x) (void)hipFree(d.x);
+  d = DeltaPack();
+}
+
+// The delta pack of the T list over partition 0, from its 32-bit pack:
+// a histogram of the deltas picks the table (the 14 commonest below 256), a
+// histogram of each block's most exceptions per range picks E (the fewest
+// index bytes: a fallback block reads its 8 KiB of 32-bit words on top of
+// its delta words), then the
+// words.  Built for entry-cut partitions that fill the device at K = 1 (so
+// that the K = 1 launches use partition 0); none when the device has no room.
+static int build_delta_pack(Plan *p, int32_t *scratch, hipStream_t st) {
+  const Partition &pt = p->partT[0];
+  const Pack &k = p->packT[0];
+  DeltaPack &dp = p->dpack;
+  if (!k.pk || k.packed == 0 || pt.rowsInKey || pt.maxT > kCap || pt.nblocks < kFillPerCu * p->cus) return 0;
+  uint32_t *hist = nullptr;
+  if (hipMalloc(&hist, sizeof(uint32_t) * 256) != hipSuccess) { (void)hipGetLastError(); return 0; }
+  int rc = (int)hipMemsetAsync(hist, 0, sizeof(uint32_t) * 256, st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_delta_hist, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, k.pk, k.base, k.stride, hist);
+    rc = (int)hipGetLastError();
+  }
+  std::vector<uint32_t> h(256, 0u);
+  if (!rc) rc = (int)hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * 256, hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  std::vector<int> cand;
+  for (int d = 1; d < 256; ++d)
+    if (h[d]) cand.push_back(d);
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return h[a] > h[b]; });
+  uint8_t tb[16] = {0};
+  for (int c = 1; c < (int)kDEscape && c - 1 < (int)cand.size(); ++c) tb[c] = (uint8_t)cand[c - 1];
+  for (int i = 0; i < 4; ++i)
+    dp.tbl[i] = (uint32_t)tb[4 * i] | ((uint32_t)tb[4 * i + 1] << 8) | ((uint32_t)tb[4 * i + 2] << 16) |
+                ((uint32_t)tb[4 * i + 3] << 24);
+  const uint4 tbv = make_uint4(dp.tbl[0], dp.tbl[1], dp.tbl[2], dp.tbl[3]);
+  // blocks by their most exceptions in one range: [0, kDMaxE], more
+  if (!rc) rc = (int)hipMemsetAsync(hist, 0, sizeof(uint32_t) * (kDMaxE + 2), st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_delta_count, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, k.pk, k.base, k.stride, tbv, hist);
+    rc = (int)hipGetLastError();
+  }
+  if (!rc) rc = (int)hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * (kDMaxE + 2), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  (void)hipFree(hist);
+  if (rc) return rc;
+  double best = -1.0;
+  for (int E = 4; E <= kDMaxE; E += 4) {
+    double over = k.plain;
+    for (int m = E + 1; m <= kDMaxE + 1; ++m) over += h[m];
+    const double bytes = (double)pt.nblocks * kDRanges * E * 4.0 + over * 4.0 * kCap;
+    if (best < 0.0 || bytes < best) { best = bytes; dp.E = E; }
+  }
+  const size_t nw = (size_t)pt.nblocks * kCap, nx = (size_t)pt.nblocks * kDRanges * dp.E;
+  if (hipMalloc(&dp.w, sizeof(uint16_t) * nw) != hipSuccess || hipMalloc(&dp.x, sizeof(uint32_t) * nx) != hipSuccess) {
+    (void)hipGetLastError();  // no room: the 32-bit pack serves
+    if (dp.w) (void)hipFree(dp.w);
+    dp = DeltaPack();
+    return 0;
+  }
+  rc = (int)hipMemsetAsync(dp.w, 0, sizeof(uint16_t) * nw, st);
+  if (!rc) rc = (int)hipMemsetAsync(dp.x, 0, sizeof(uint32_t) * nx, st);
+  if (!rc) rc = (int)hipMemsetAsync(scratch, 0, sizeof(int32_t), st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_build_delta, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, k.pk, k.base, k.stride, tbv, dp.E,
+                       dp.w, dp.x, scratch);
+    rc = (int)hipGetLastError();
+  }
+  int32_t fb = 0;
+  if (!rc) rc = (int)hipMemcpyAsync(&fb, scratch, sizeof(fb), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  dp.fallback = fb;
+  // mostly fallback blocks (columns without short runs): the 32-bit pack serves
+  if (!rc && 4 * (int64_t)fb > pt.nblocks) free_delta(dp);
+  return rc;
+}
+
 static void free_pack(Pack &k) {
   if (k.pk) (void)hipFree(k.pk);
   if (k.base) (void)hipFree(k.base);
   k = Pack();
+}
+
+// Device bytes a plan owns (arrays shared between its members counted once).
+static int64_t plan_bytes(const Plan *p) {
+  std::vector<const void *> seen;
+  int64_t total = 0;
+  auto add = [&](const void *ptr, int64_t bytes) {
+    if (!ptr || std::find(seen.begin(), seen.end(), ptr) != seen.end()) return;
+    seen.push_back(ptr);
+    total += bytes;
+  };
+  auto part = [&](const Partition &x) {
+    add(x.d, (int64_t)(x.nblocks + 1) * (int64_t)sizeof(int4));
+    add(x.order, (int64_t)x.nblocks * (int64_t)sizeof(int));
+  };
+  auto pack = [&](const Pack &k, const Partition &x) {
+    add(k.pk, (int64_t)k.stride * x.nblocks * 4);
+    add(k.base, (int64_t)x.nblocks * (int64_t)sizeof(int4));
+  };
+  for (int j = 0; j < p->nparts; ++j) {
+    part(p->partT[j]);
+    part(p->partN[j]);
+    part(p->partS[j]);
+  }
+  for (int j = 0; j < kPackParts && j < p->nparts; ++j) pack(p->packT[j], p->partT[j]);
+  if (p->nparts > 0) {
+    pack(p->packNT, p->partT[0]);
+    pack(p->packNN, p->partN[0]);
+    pack(p->packS, p->partS[0]);
+  }
+  for (int i = 0; i < 2; ++i) {
+    part(p->part3[i]);
+    pack(p->pack3[i], p->part3[i]);
+  }
+  if (p->dpack.w) {
+    add(p->dpack.w, (int64_t)p->partT[0].nblocks * kCap * 2);
+    add(p->dpack.x, (int64_t)p->partT[0].nblocks * kDRanges * p->dpack.E * 4);
+  }
+  add(p->colRowT, (int64_t)p->Ncols * 4);
+  add(p->colRowN, (int64_t)p->Ncols * 4);
+  add(p->colOwner, (int64_t)p->Ncols * 4);
+  add(p->nMask, ((int64_t)p->Ncols + 31) / 32 * 4);
+  return total;
 }
 
 static void free_plan(Plan *p) {
@@ -2813,6 +3248,7 @@ static void free_plan(Plan *p) {
   if (nn) free_pack(p->packNN);
   if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
+  free_delta(p->dpack);
   for (int i = 0; i < 2; ++i) {
     free_pack(p->pack3[i]);
     if (p->part3[i].d) (void)hipFree(p->part3[i].d);
@@ -2970,6 +3406,7 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
       if (!rc) rc = build_pack(p->part3[i], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->pack3[i]);
     }
   }
+  if (!rc && cutTOnEntries) rc = build_delta_pack(p, scratch, st);
   (void)hipFree(scratch);
   if (!rc && cutTOnEntries) rc = tile_schedule(p, st);
   if (rc) { free_plan(p); return rc; }
@@ -2999,6 +3436,9 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->tileD = p->nparts ? p->partT[0].tileD : 0;
   info->colsT = p->colRowT != nullptr;
   info->colsN = p->colRowN != nullptr;
+  info->deltaE = p->dpack.w ? p->dpack.E : 0;
+  info->deltaFallback = p->dpack.w ? p->dpack.fallback : 0;
+  info->bytes = plan_bytes(p);
   return OGS_SUCCESS;
 }
 

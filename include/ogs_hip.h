@@ -136,11 +136,16 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * block id order).  Plan creation picks a tiled order, per kernel, only where
  * it measures faster on scratch values.  colsT / colsN: 1 where the plan
  * holds each column's row for that list (a list with at least Ncols/2
- * entries), so that Scatter can run by columns (see ogs_hip_scatter). */
+ * entries), so that Scatter can run by columns (see ogs_hip_scatter).
+ * deltaE: exception words per 512-entry range of the K=1 GatherScatter's
+ * 16-bit delta pack (0 = none; the 32-bit pack serves), deltaFallback: its
+ * blocks that run from the 32-bit pack.  bytes: device memory the plan owns
+ * (partitions, packs, column maps, block orders), beside the caller's CSR. */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
-  int32_t colsT, colsN;
+  int32_t colsT, colsN, deltaE, deltaFallback;
+  int64_t bytes;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
 

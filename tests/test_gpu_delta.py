@@ -1,0 +1,170 @@
+"""GPU parity of the K = 1 one-list GatherScatter over the plan's 16-bit
+delta pack (k_gs_sym_delta): bit-exact against the oracle on meshes large
+enough for the pack (partition 0 fills the device), for every type and op,
+Unsigned and Signed (Sym), with fallback blocks (columns without short runs)
+mixed in, and with the pack dropped when most blocks would fall back.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TYPES = {0: np.float32, 1: np.float64, 2: np.int32, 3: np.int64}
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dawgs_amd
+    return dawgs_amd
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.dtype == b.dtype and a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def values(rng, dt, n):
+    """Uniform values with 1% signed zeros and NaN among them (no
+    infinities: a NaN made by inf - inf or inf * 0 has a sign that differs
+    between the host and the GPU, while an input NaN passes through both)."""
+    if np.issubdtype(dt, np.floating):
+        v = rng.uniform(-1, 1, n).astype(dt)
+        at = rng.random(n) < 0.01
+        v[at] = np.array([0.0, -0.0, np.nan], dtype=dt)[rng.integers(0, 3, int(at.sum()))]
+        return v
+    return rng.integers(-2 ** 20, 2 ** 20 + 1, n).astype(dt)
+
+
+def setup(D, ids, kind=O.UNSIGNED, unique=False):
+    g = D.ogs_t()
+    d = ids.copy()
+    g.Setup(len(d), d, kind, D.Auto, unique, False)
+    return g, d
+
+
+BOXES = [(16, 16, 16, 7), (36, 32, 32, 3), (22, 22, 22, 5), (104, 100, 96, 1)]
+_CACHE = {}
+
+
+def unsigned_mesh(D, box):
+    if box not in _CACHE:
+        ids = O.make_ids(*box)
+        g, _ = setup(D, ids)
+        c = g.csr()
+        _CACHE[box] = (g, O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids)), len(ids))
+    return _CACHE[box]
+
+
+@pytest.mark.parametrize("box", BOXES, ids=lambda b: "x".join(map(str, b)))
+def test_delta_pack_built(D, box):
+    g, _, N = unsigned_mesh(D, box)
+    info = g.plan.info()
+    assert info["deltaE"] > 0 and info["deltaFallback"] <= info["nblocksK1"] // 8, info
+    assert info["nblocksK1"] >= 4 * 256  # partition 0 fills the device: the K = 1 launch uses it
+    assert 0 < info["bytes"]
+
+
+@pytest.mark.parametrize("box", BOXES, ids=lambda b: "x".join(map(str, b)))
+@pytest.mark.parametrize("ty", list(TYPES))
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_delta_gather_scatter(torch, D, box, ty, op):
+    g, ref, N = unsigned_mesh(D, box)
+    dt = TYPES[ty]
+    rng = np.random.default_rng(hash((box, ty, op)) % 2 ** 32)
+    v = values(rng, dt, N)
+    want = ref.gather_scatter(v.copy(), 1, op, O.SYM)
+    d = torch.from_numpy(v.copy()).cuda()
+    g.GatherScatter(d, 1, op, D.Sym)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), want), (box, np.dtype(dt).name, op)
+
+
+@pytest.mark.parametrize("ty,op", [(O.FLOAT_C, 0), (O.BF16, 0), (O.BF16, 2), (O.FLOAT_C, 3)])
+def test_delta_precision_types(torch, D, ty, op):
+    box = BOXES[0]
+    g, ref, N = unsigned_mesh(D, box)
+    rng = np.random.default_rng(5 + ty + op)
+    f = (rng.uniform(-1, 1, N) * 10.0 ** rng.uniform(-3, 3, N)).astype(np.float32)
+    if ty == O.BF16:
+        v = O.float_to_bf16(f)
+        d = torch.from_numpy(v.view(np.int16).copy()).cuda().view(torch.bfloat16)
+    else:
+        v = f
+        d = torch.from_numpy(v.copy()).cuda()
+    want = ref.gather_scatter(v.copy(), 1, op, O.SYM, type=ty)
+    g.GatherScatter(d, 1, op, D.Sym, type=(D.FloatC if ty == O.FLOAT_C else None))
+    torch.cuda.synchronize()
+    got = d.view(torch.int16).cpu().numpy().view(np.uint16) if ty == O.BF16 else d.cpu().numpy()
+    assert same(got, want)
+
+
+@pytest.mark.parametrize("unique,flag", [(True, 0.0), (False, 0.3)])
+def test_delta_signed_sym(torch, D, unique, flag):
+    box = BOXES[1]
+    rng = np.random.default_rng(11)
+    ids = O.make_ids(*box).copy()
+    ids[rng.random(len(ids)) < 0.05] = 0
+    if flag:
+        neg = rng.random(len(ids)) < flag
+        ids[neg] = -ids[neg]
+    o = O.OracleOgs(ids, O.SIGNED, unique)
+    g, _ = setup(D, ids, O.SIGNED, unique)
+    assert g.plan.info()["deltaE"] > 0
+    for op in (0, 2):
+        v = values(rng, np.float64, len(ids))
+        want = o.op_gather_scatter(v.copy(), 1, op, O.SYM)
+        d = torch.from_numpy(v.copy()).cuda()
+        g.GatherScatter(d, 1, op, D.Sym)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), want), op
+
+
+def shuffled(box, frac, seed):
+    """Structured ids whose last `frac` of local positions are shuffled among
+    themselves: the blocks of those rows hold columns without short runs."""
+    ids = O.make_ids(*box).copy()
+    n = len(ids)
+    a = int(n * (1 - frac))
+    rng = np.random.default_rng(seed)
+    ids[a:] = ids[a:][rng.permutation(n - a)]
+    return ids
+
+
+def test_delta_fallback_blocks(torch, D):
+    ids = shuffled((36, 32, 32, 3), 0.08, 3)
+    g, _ = setup(D, ids)
+    info = g.plan.info()
+    assert info["deltaE"] > 0 and 0 < info["deltaFallback"] < info["nblocksK1"] // 4, info
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids))
+    rng = np.random.default_rng(4)
+    for dt, op in ((np.float64, 0), (np.int32, 1), (np.float32, 3)):
+        v = values(rng, dt, len(ids))
+        want = ref.gather_scatter(v.copy(), 1, op, O.SYM)
+        d = torch.from_numpy(v.copy()).cuda()
+        g.GatherScatter(d, 1, op, D.Sym)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), want), (np.dtype(dt).name, op)
+
+
+def test_delta_dropped_when_mostly_fallback(torch, D):
+    ids = shuffled((36, 32, 32, 3), 1.0, 9)
+    g, _ = setup(D, ids)
+    assert g.plan.info()["deltaE"] == 0
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids))
+    v = np.random.default_rng(8).uniform(-1, 1, len(ids))
+    want = ref.gather_scatter(v.copy(), 1, O.ADD, O.SYM)
+    d = torch.from_numpy(v.copy()).cuda()
+    g.GatherScatter(d, 1, D.Add, D.Sym)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), want)
