@@ -387,6 +387,26 @@ def run_precision(args) -> int:
     return 0
 
 
+def idle_probe(call, stream, gaps_ms=(0, 5, 20, 100, 500), n=120) -> dict:
+    """Per-call times (us) after each idle gap: `n` calls with a HIP event
+    between consecutive launches, reported as means of runs of 10."""
+    import torch
+    out = {}
+    for gap in gaps_ms:
+        torch.cuda.synchronize()
+        time.sleep(gap / 1e3)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+        ev[0].record(stream)
+        for i in range(n):
+            call()
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        t = [ev[i].elapsed_time(ev[i + 1]) * 1e3 for i in range(n)]
+        out[f"{gap}ms"] = [round(sum(t[i:i + 10]) / 10, 1) for i in range(0, n, 10)]
+        log(f"idle {gap:4d} ms, then per 10 calls (us): " + " ".join(f"{x:.1f}" for x in out[f"{gap}ms"]))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,6 +438,10 @@ def main() -> int:
                     help="after the timed region, time a plain device copy of the same byte count "
                          "(the practical streaming ceiling, and a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--idle-probe", action="store_true",
+                    help="after the timed region, time the GatherScatter call by call after idle gaps of "
+                         "0, 5, 20, 100 and 500 ms (means of runs of 10 calls; how the board's power "
+                         "management shapes the first ~100 calls after a pause)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -473,7 +497,8 @@ def main() -> int:
     ms = ev0.elapsed_time(ev1) / args.steps
     info = ogs.plan.info()
     log(f"setup {time.perf_counter() - t0:.2f} s (incl. timing): N={N} NgatherGlobal={ogs.NgatherGlobal} "
-        f"nnz={ogs.nnzT} blocks={info['nblocksK1']} W={info['widthK1']} maxRow={info['maxRowT']}")
+        f"nnz={ogs.nnzT} blocks={info['nblocksK1']} W={info['widthK1']} maxRow={info['maxRowT']} "
+        f"deltaE={info['deltaE']} deltaFallback={info['deltaFallback']} planMB={info['bytes'] / 1e6:.0f}")
 
     B = algorithmic_bytes(ogs.nnzT, ogs.NrowsT, 1, 8)
     achieved = B / (ms * 1e-3) / 1e9
@@ -511,6 +536,8 @@ def main() -> int:
                                   "GBps": round(16 * n / (cms * 1e-3) / 1e9, 1)}
         del src, dst
         log(f"copy probe: {16 * n} B in {cms * 1e3:.1f} us = {16 * n / (cms * 1e-3) / 1e9:.0f} GB/s")
+    if args.idle_probe:
+        roofline["idle_probe"] = idle_probe(lambda: ogs.GatherScatter(q, 1, D.Add, D.Sym), stream)
     log(f"GPU: {ms * 1e3:.1f} us/call, {achieved:.0f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s), "
         f"{ogs.NgatherGlobal / (ms * 1e-3):.4g} DOFS/s")
 

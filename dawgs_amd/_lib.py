@@ -19,7 +19,8 @@ _I32P = C.POINTER(C.c_int32)
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
-                 "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN", "deltaE", "deltaFallback")] + \
+                 "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN", "deltaE", "deltaFallback",
+                 "ownerMap", "nMask")] + \
                [("bytes", C.c_int64)]
 
 
@@ -51,6 +52,7 @@ class HaloInfo(C.Structure):
 SIGNATURES = {
     "ogs_hip_error_string": (C.c_char_p, [C.c_int]),
     "ogs_hip_plan_create": (C.c_int, [C.POINTER(_P), _I32, _I32, _I32, _P, _P, _P, _P, _P]),
+    "ogs_hip_plan_retain": (C.c_int, [_P]),
     "ogs_hip_plan_destroy": (C.c_int, [_P]),
     "ogs_hip_gather_scatter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P]),
     "ogs_hip_gather": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P, _P]),

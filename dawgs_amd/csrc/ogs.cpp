@@ -17,44 +17,47 @@ namespace libp {
 namespace ogs {
 
 template <typename T>
-void ogsOperator_t::GatherScatter(deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s) {
+void ogsOperator_t::GatherScatter(deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s) const {
   const int rc = ogs_hip_gather_scatter(plan_, (int)ogsType<T>::get(), (int)op, (int)trans, k, o_v.ptr(), s);
   LIBP_ABORT("ogs_hip_gather_scatter: " << ogs_hip_error_string(rc), rc != 0, rc);
 }
 
 template <typename T>
-void ogsOperator_t::Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s) {
+void ogsOperator_t::Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, int k, Op op, Transpose trans,
+                           hipStream_t s) const {
   const int rc = ogs_hip_gather(plan_, (int)ogsType<T>::get(), (int)op, (int)trans, k, o_gv.ptr(), o_v.ptr(), s);
   LIBP_ABORT("ogs_hip_gather: " << ogs_hip_error_string(rc), rc != 0, rc);
 }
 
 template <typename T>
-void ogsOperator_t::Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, int k, Transpose trans, hipStream_t s) {
+void ogsOperator_t::Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, int k, Transpose trans, hipStream_t s) const {
   const int rc = ogs_hip_scatter(plan_, (int)ogsType<T>::get(), (int)trans, k, o_v.ptr(), o_gv.ptr(), s);
   LIBP_ABORT("ogs_hip_scatter: " << ogs_hip_error_string(rc), rc != 0, rc);
 }
 
 // ---- GatherScatter (ogs.cpp:39-123) ----
 template <typename T>
-void ogs_t::GatherScatter(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
-  GatherScatterStart(o_v, k, op, trans);
-  GatherScatterFinish(o_v, k, op, trans);
+void ogs_t::GatherScatter(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans, hipStream_t s) const {
+  GatherScatterStart(o_v, k, op, trans, s);
+  GatherScatterFinish(o_v, k, op, trans, s);
 }
 
 template <typename T>
-void ogs_t::GatherScatterStart(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+void ogs_t::GatherScatterStart(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                               hipStream_t s) const {
   LIBP_ABORT("ogs handle is not set up.", !gatherLocal, OGS_E_ARG);
   LIBP_ABORT("k must be positive", k < 1, OGS_E_ARG);
   // halo gather into the exchange workspace: zero rows at one rank (ogs.cpp:58)
-  gatherHalo->Gather(deviceMemory<T>(), o_v, k, op, trans, platform.stream);
+  gatherHalo->Gather(deviceMemory<T>(), o_v, k, op, trans, s);
 }
 
 template <typename T>
-void ogs_t::GatherScatterFinish(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+void ogs_t::GatherScatterFinish(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                                hipStream_t s) const {
   // the local operator runs `op`, as the host path does (ogsOperator.cpp:362-378);
   // the reference's device overload always adds (ogsOperator.cpp:399-430)
-  gatherLocal->GatherScatter(o_v, k, op, trans, platform.stream);  // ogs.cpp:91
-  gatherHalo->Scatter(o_v, deviceMemory<T>(), k, trans, platform.stream);  // ogs.cpp:122, zero rows
+  gatherLocal->GatherScatter(o_v, k, op, trans, s);  // ogs.cpp:91
+  gatherHalo->Scatter(o_v, deviceMemory<T>(), k, trans, s);  // ogs.cpp:122, zero rows
 }
 
 template <typename T>
@@ -67,23 +70,26 @@ void ogs_t::GatherScatter(memory<T> v, const int k, const Op op, const Transpose
 
 // ---- Gather (ogs.cpp:202-374) ----
 template <typename T>
-void ogs_t::Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
-  GatherStart(o_gv, o_v, k, op, trans);
-  GatherFinish(o_gv, o_v, k, op, trans);
+void ogs_t::Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                   hipStream_t s) const {
+  GatherStart(o_gv, o_v, k, op, trans, s);
+  GatherFinish(o_gv, o_v, k, op, trans, s);
 }
 
 template <typename T>
-void ogs_t::GatherStart(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+void ogs_t::GatherStart(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                        hipStream_t s) const {
   AssertGatherDefined();  // ogs.cpp:218
   LIBP_ABORT("k must be positive", k < 1, OGS_E_ARG);
   // Trans: halo gather into the workspace; else into o_gv + k*NlocalT (ogs.cpp:222-251); zero rows here
-  gatherHalo->Gather(o_gv + (size_t)k * NlocalT, o_v, k, op, trans, platform.stream);
+  gatherHalo->Gather(o_gv + (size_t)k * NlocalT, o_v, k, op, trans, s);
 }
 
 template <typename T>
-void ogs_t::GatherFinish(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+void ogs_t::GatherFinish(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                         hipStream_t s) const {
   AssertGatherDefined();  // ogs.cpp:260
-  gatherLocal->Gather(o_gv, o_v, k, op, trans, platform.stream);  // ogs.cpp:265
+  gatherLocal->Gather(o_gv, o_v, k, op, trans, s);  // ogs.cpp:265
 }
 
 template <typename T>
@@ -97,24 +103,27 @@ void ogs_t::Gather(memory<T> gv, const memory<T> v, const int k, const Op op, co
 
 // ---- Scatter (ogs.cpp:392-561) ----
 template <typename T>
-void ogs_t::Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans) {
-  ScatterStart(o_v, o_gv, k, trans);
-  ScatterFinish(o_v, o_gv, k, trans);
+void ogs_t::Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans,
+                    hipStream_t s) const {
+  ScatterStart(o_v, o_gv, k, trans, s);
+  ScatterFinish(o_v, o_gv, k, trans, s);
 }
 
 template <typename T>
-void ogs_t::ScatterStart(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans) {
-  (void)o_v; (void)o_gv; (void)trans;
+void ogs_t::ScatterStart(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans,
+                         hipStream_t s) const {
+  (void)o_v; (void)o_gv; (void)trans; (void)s;
   AssertGatherDefined();  // ogs.cpp:406
   LIBP_ABORT("k must be positive", k < 1, OGS_E_ARG);
   // NoTrans: the halo tail of o_gv goes to the exchange (ogs.cpp:410-441); empty here
 }
 
 template <typename T>
-void ogs_t::ScatterFinish(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans) {
+void ogs_t::ScatterFinish(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans,
+                          hipStream_t s) const {
   AssertGatherDefined();  // ogs.cpp:449
-  gatherLocal->Scatter(o_v, o_gv, k, trans, platform.stream);  // ogs.cpp:454
-  gatherHalo->Scatter(o_v, o_gv + (size_t)k * NlocalT, k, trans, platform.stream);  // ogs.cpp:483/486, zero rows
+  gatherLocal->Scatter(o_v, o_gv, k, trans, s);  // ogs.cpp:454
+  gatherHalo->Scatter(o_v, o_gv + (size_t)k * NlocalT, k, trans, s);  // ogs.cpp:483/486, zero rows
 }
 
 template <typename T>
@@ -150,12 +159,18 @@ void ogs_t::SetupGlobalToLocalMapping(dlong *GlobalToLocal) {
   platform.finish();
 }
 
-#define OGS_INSTANTIATE(T)                                                                           \
-  template void ogs_t::GatherScatter(deviceMemory<T>, const int, const Op, const Transpose);        \
-  template void ogs_t::GatherScatter(memory<T>, const int, const Op, const Transpose);              \
-  template void ogs_t::Gather(deviceMemory<T>, deviceMemory<T>, const int, const Op, const Transpose); \
-  template void ogs_t::Gather(memory<T>, const memory<T>, const int, const Op, const Transpose);     \
-  template void ogs_t::Scatter(deviceMemory<T>, deviceMemory<T>, const int, const Transpose);        \
+#define OGS_INSTANTIATE(T)                                                                                       \
+  template void ogs_t::GatherScatter(deviceMemory<T>, const int, const Op, const Transpose, hipStream_t) const;   \
+  template void ogs_t::GatherScatterStart(deviceMemory<T>, const int, const Op, const Transpose, hipStream_t) const; \
+  template void ogs_t::GatherScatterFinish(deviceMemory<T>, const int, const Op, const Transpose, hipStream_t) const; \
+  template void ogs_t::GatherScatter(memory<T>, const int, const Op, const Transpose);                          \
+  template void ogs_t::Gather(deviceMemory<T>, deviceMemory<T>, const int, const Op, const Transpose, hipStream_t) const; \
+  template void ogs_t::GatherStart(deviceMemory<T>, deviceMemory<T>, const int, const Op, const Transpose, hipStream_t) const; \
+  template void ogs_t::GatherFinish(deviceMemory<T>, deviceMemory<T>, const int, const Op, const Transpose, hipStream_t) const; \
+  template void ogs_t::Gather(memory<T>, const memory<T>, const int, const Op, const Transpose);                 \
+  template void ogs_t::Scatter(deviceMemory<T>, deviceMemory<T>, const int, const Transpose, hipStream_t) const; \
+  template void ogs_t::ScatterStart(deviceMemory<T>, deviceMemory<T>, const int, const Transpose, hipStream_t) const; \
+  template void ogs_t::ScatterFinish(deviceMemory<T>, deviceMemory<T>, const int, const Transpose, hipStream_t) const; \
   template void ogs_t::Scatter(memory<T>, const memory<T>, const int, const Transpose);
 OGS_INSTANTIATE(float)
 OGS_INSTANTIATE(double)
@@ -280,40 +295,41 @@ int ogs_setup_global_to_local(ogs_handle_t h, int32_t *GlobalToLocal) {
 
 int ogs_gather_scatter(ogs_handle_t h, int type, int op, int trans, int32_t K, void *d_v, void *stream) {
   if (!h || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;
-  ogs::ogs_t &o = h->ogs;
-  o.platform.stream = static_cast<hipStream_t>(stream);
+  const ogs::ogs_t &o = h->ogs;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
   return by_type(type, [&](auto tag) {
     using T = decltype(tag);
     return guarded([&] {
-      o.GatherScatter(deviceMemory<T>(static_cast<T *>(d_v), (size_t)o.N * K), K, (ogs::Op)op, (ogs::Transpose)trans);
+      o.GatherScatter(deviceMemory<T>(static_cast<T *>(d_v), (size_t)o.N * K), K, (ogs::Op)op, (ogs::Transpose)trans,
+                      s);
     });
   });
 }
 
 int ogs_gather(ogs_handle_t h, int type, int op, int trans, int32_t K, void *d_gv, const void *d_v, void *stream) {
   if (!h || bad_op(op) || bad_trans(trans) || K < 1) return OGS_E_ARG;
-  ogs::ogs_t &o = h->ogs;
-  o.platform.stream = static_cast<hipStream_t>(stream);
+  const ogs::ogs_t &o = h->ogs;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
   return by_type(type, [&](auto tag) {
     using T = decltype(tag);
     return guarded([&] {
       o.Gather(deviceMemory<T>(static_cast<T *>(d_gv), (size_t)o.NlocalT * K),
                deviceMemory<T>(const_cast<T *>(static_cast<const T *>(d_v)), (size_t)o.N * K), K, (ogs::Op)op,
-               (ogs::Transpose)trans);
+               (ogs::Transpose)trans, s);
     });
   });
 }
 
 int ogs_scatter(ogs_handle_t h, int type, int trans, int32_t K, void *d_v, const void *d_gv, void *stream) {
   if (!h || bad_trans(trans) || K < 1) return OGS_E_ARG;
-  ogs::ogs_t &o = h->ogs;
-  o.platform.stream = static_cast<hipStream_t>(stream);
+  const ogs::ogs_t &o = h->ogs;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
   return by_type(type, [&](auto tag) {
     using T = decltype(tag);
     return guarded([&] {
       o.Scatter(deviceMemory<T>(static_cast<T *>(d_v), (size_t)o.N * K),
                 deviceMemory<T>(const_cast<T *>(static_cast<const T *>(d_gv)), (size_t)o.NlocalT * K), K,
-                (ogs::Transpose)trans);
+                (ogs::Transpose)trans, s);
     });
   });
 }

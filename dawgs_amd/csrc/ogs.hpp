@@ -192,11 +192,11 @@ class ogsOperator_t {
   ogs_hip_plan_t plan() const { return plan_; }
 
   template <typename T>
-  void Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s);
+  void Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s) const;
   template <typename T>
-  void Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, int k, Transpose trans, hipStream_t s);
+  void Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, int k, Transpose trans, hipStream_t s) const;
   template <typename T>
-  void GatherScatter(deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s);
+  void GatherScatter(deviceMemory<T> o_v, int k, Op op, Transpose trans, hipStream_t s) const;
 
  private:
   ogs_hip_plan_t plan_ = nullptr;
@@ -223,19 +223,58 @@ class ogs_t {
              const bool verbose, platform_t &_platform);
   void Free();
 
-  template <typename T> void GatherScatter(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans);
-  template <typename T> void GatherScatterStart(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans);
-  template <typename T> void GatherScatterFinish(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans);
+  // The reference's signatures launch on platform.stream; each has an
+  // overload taking the stream explicitly, which never writes the object, so
+  // several host threads may share one ogs_t on different streams.
+  template <typename T> void GatherScatter(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+    GatherScatter(o_v, k, op, trans, platform.stream);
+  }
+  template <typename T> void GatherScatterStart(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+    GatherScatterStart(o_v, k, op, trans, platform.stream);
+  }
+  template <typename T> void GatherScatterFinish(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+    GatherScatterFinish(o_v, k, op, trans, platform.stream);
+  }
+  template <typename T> void GatherScatter(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                                           hipStream_t s) const;
+  template <typename T> void GatherScatterStart(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                                                hipStream_t s) const;
+  template <typename T> void GatherScatterFinish(deviceMemory<T> o_v, const int k, const Op op, const Transpose trans,
+                                                 hipStream_t s) const;
   template <typename T> void GatherScatter(memory<T> v, const int k, const Op op, const Transpose trans);
 
-  template <typename T> void Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans);
-  template <typename T> void GatherStart(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans);
-  template <typename T> void GatherFinish(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans);
+  template <typename T> void Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+    Gather(o_gv, o_v, k, op, trans, platform.stream);
+  }
+  template <typename T> void GatherStart(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+    GatherStart(o_gv, o_v, k, op, trans, platform.stream);
+  }
+  template <typename T> void GatherFinish(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op, const Transpose trans) {
+    GatherFinish(o_gv, o_v, k, op, trans, platform.stream);
+  }
+  template <typename T> void Gather(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op,
+                                    const Transpose trans, hipStream_t s) const;
+  template <typename T> void GatherStart(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op,
+                                         const Transpose trans, hipStream_t s) const;
+  template <typename T> void GatherFinish(deviceMemory<T> o_gv, deviceMemory<T> o_v, const int k, const Op op,
+                                          const Transpose trans, hipStream_t s) const;
   template <typename T> void Gather(memory<T> gv, const memory<T> v, const int k, const Op op, const Transpose trans);
 
-  template <typename T> void Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans);
-  template <typename T> void ScatterStart(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans);
-  template <typename T> void ScatterFinish(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans);
+  template <typename T> void Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans) {
+    Scatter(o_v, o_gv, k, trans, platform.stream);
+  }
+  template <typename T> void ScatterStart(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans) {
+    ScatterStart(o_v, o_gv, k, trans, platform.stream);
+  }
+  template <typename T> void ScatterFinish(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans) {
+    ScatterFinish(o_v, o_gv, k, trans, platform.stream);
+  }
+  template <typename T> void Scatter(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k, const Transpose trans,
+                                     hipStream_t s) const;
+  template <typename T> void ScatterStart(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k,
+                                          const Transpose trans, hipStream_t s) const;
+  template <typename T> void ScatterFinish(deviceMemory<T> o_v, deviceMemory<T> o_gv, const int k,
+                                           const Transpose trans, hipStream_t s) const;
   template <typename T> void Scatter(memory<T> v, const memory<T> gv, const int k, const Transpose trans);
 
   // ogsSetup.cpp:867-886: GlobalToLocal[n] = the gathered index (row) of node
@@ -280,14 +319,22 @@ class halo_t {
   void SetupFromGather(ogs_t &ogs);
   void Free();
 
-  template <typename T> void Exchange(deviceMemory<T> o_v, const int k);
-  template <typename T> void ExchangeStart(deviceMemory<T> o_v, const int k);
-  template <typename T> void ExchangeFinish(deviceMemory<T> o_v, const int k);
+  // platform.stream, as in ogs_t; the overloads taking a stream never write
+  // the object.
+  template <typename T> void Exchange(deviceMemory<T> o_v, const int k) { Exchange(o_v, k, platform.stream); }
+  template <typename T> void ExchangeStart(deviceMemory<T> o_v, const int k) { ExchangeStart(o_v, k, platform.stream); }
+  template <typename T> void ExchangeFinish(deviceMemory<T> o_v, const int k) { ExchangeFinish(o_v, k, platform.stream); }
+  template <typename T> void Exchange(deviceMemory<T> o_v, const int k, hipStream_t s) const;
+  template <typename T> void ExchangeStart(deviceMemory<T> o_v, const int k, hipStream_t s) const;
+  template <typename T> void ExchangeFinish(deviceMemory<T> o_v, const int k, hipStream_t s) const;
   template <typename T> void Exchange(memory<T> v, const int k);
 
-  template <typename T> void Combine(deviceMemory<T> o_v, const int k);
-  template <typename T> void CombineStart(deviceMemory<T> o_v, const int k);
-  template <typename T> void CombineFinish(deviceMemory<T> o_v, const int k);
+  template <typename T> void Combine(deviceMemory<T> o_v, const int k) { Combine(o_v, k, platform.stream); }
+  template <typename T> void CombineStart(deviceMemory<T> o_v, const int k) { CombineStart(o_v, k, platform.stream); }
+  template <typename T> void CombineFinish(deviceMemory<T> o_v, const int k) { CombineFinish(o_v, k, platform.stream); }
+  template <typename T> void Combine(deviceMemory<T> o_v, const int k, hipStream_t s) const;
+  template <typename T> void CombineStart(deviceMemory<T> o_v, const int k, hipStream_t s) const;
+  template <typename T> void CombineFinish(deviceMemory<T> o_v, const int k, hipStream_t s) const;
   template <typename T> void Combine(memory<T> v, const int k);
 
  private:

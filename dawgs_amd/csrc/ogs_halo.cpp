@@ -82,23 +82,23 @@ void halo_t::Check(const int k) const {
 
 // ---- Exchange (ogsHalo.cpp:39-196) ----
 template <typename T>
-void halo_t::Exchange(deviceMemory<T> o_v, const int k) {
-  ExchangeStart(o_v, k);
-  ExchangeFinish(o_v, k);
+void halo_t::Exchange(deviceMemory<T> o_v, const int k, hipStream_t s) const {
+  ExchangeStart(o_v, k, s);
+  ExchangeFinish(o_v, k, s);
 }
 
 template <typename T>
-void halo_t::ExchangeStart(deviceMemory<T> o_v, const int k) {
+void halo_t::ExchangeStart(deviceMemory<T> o_v, const int k, hipStream_t s) const {
   Check(k);
   // gather the NhaloP owned halo values (zero of them), ogsHalo.cpp:47-61
-  if (!gathered_halo) gatherHalo->Gather(deviceMemory<T>(), o_v, k, Add, NoTrans, platform.stream);
+  if (!gathered_halo) gatherHalo->Gather(deviceMemory<T>(), o_v, k, Add, NoTrans, s);
 }
 
 template <typename T>
-void halo_t::ExchangeFinish(deviceMemory<T> o_v, const int k) {
+void halo_t::ExchangeFinish(deviceMemory<T> o_v, const int k, hipStream_t s) const {
   Check(k);
   // scatter the received Nhalo values (zero of them), ogsHalo.cpp:99-110
-  if (!gathered_halo) gatherHalo->Scatter(o_v, deviceMemory<T>(), k, NoTrans, platform.stream);
+  if (!gathered_halo) gatherHalo->Scatter(o_v, deviceMemory<T>(), k, NoTrans, s);
 }
 
 template <typename T>
@@ -109,23 +109,23 @@ void halo_t::Exchange(memory<T> v, const int k) {
 
 // ---- Combine (ogsHalo.cpp:201-358) ----
 template <typename T>
-void halo_t::Combine(deviceMemory<T> o_v, const int k) {
-  CombineStart(o_v, k);
-  CombineFinish(o_v, k);
+void halo_t::Combine(deviceMemory<T> o_v, const int k, hipStream_t s) const {
+  CombineStart(o_v, k, s);
+  CombineFinish(o_v, k, s);
 }
 
 template <typename T>
-void halo_t::CombineStart(deviceMemory<T> o_v, const int k) {
+void halo_t::CombineStart(deviceMemory<T> o_v, const int k, hipStream_t s) const {
   Check(k);
   // gather (Add, Trans) all NhaloT halo values (zero of them), ogsHalo.cpp:209-222
-  if (!gathered_halo) gatherHalo->Gather(deviceMemory<T>(), o_v, k, Add, Trans, platform.stream);
+  if (!gathered_halo) gatherHalo->Gather(deviceMemory<T>(), o_v, k, Add, Trans, s);
 }
 
 template <typename T>
-void halo_t::CombineFinish(deviceMemory<T> o_v, const int k) {
+void halo_t::CombineFinish(deviceMemory<T> o_v, const int k, hipStream_t s) const {
   Check(k);
   // scatter the combined NhaloP values back (zero of them), ogsHalo.cpp:262-272
-  if (!gathered_halo) gatherHalo->Scatter(o_v, deviceMemory<T>(), k, Trans, platform.stream);
+  if (!gathered_halo) gatherHalo->Scatter(o_v, deviceMemory<T>(), k, Trans, s);
 }
 
 template <typename T>
@@ -134,10 +134,14 @@ void halo_t::Combine(memory<T> v, const int k) {
   (void)v;
 }
 
-#define HALO_INSTANTIATE(T)                                         \
-  template void halo_t::Exchange(deviceMemory<T>, const int);       \
-  template void halo_t::Exchange(memory<T>, const int);             \
-  template void halo_t::Combine(deviceMemory<T>, const int);        \
+#define HALO_INSTANTIATE(T)                                                           \
+  template void halo_t::Exchange(deviceMemory<T>, const int, hipStream_t) const;       \
+  template void halo_t::ExchangeStart(deviceMemory<T>, const int, hipStream_t) const;  \
+  template void halo_t::ExchangeFinish(deviceMemory<T>, const int, hipStream_t) const; \
+  template void halo_t::Exchange(memory<T>, const int);                               \
+  template void halo_t::Combine(deviceMemory<T>, const int, hipStream_t) const;        \
+  template void halo_t::CombineStart(deviceMemory<T>, const int, hipStream_t) const;   \
+  template void halo_t::CombineFinish(deviceMemory<T>, const int, hipStream_t) const;  \
   template void halo_t::Combine(memory<T>, const int);
 HALO_INSTANTIATE(float)
 HALO_INSTANTIATE(double)
@@ -191,14 +195,14 @@ int halo_by_type(int type, F &&f) {
 template <bool COMBINE>
 int halo_move(halo_handle_t h, int type, int32_t K, void *d_v, void *stream) {
   if (!h || K < 1) return OGS_E_ARG;
-  ogs::halo_t &o = h->halo;
-  o.platform.stream = static_cast<hipStream_t>(stream);
+  const ogs::halo_t &o = h->halo;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
   return halo_by_type(type, [&](auto tag) {
     using T = decltype(tag);
     return halo_guarded([&] {
       deviceMemory<T> v(static_cast<T *>(d_v), (size_t)o.N * K);
-      if (COMBINE) o.Combine(v, K);
-      else o.Exchange(v, K);
+      if (COMBINE) o.Combine(v, K, s);
+      else o.Exchange(v, K, s);
     });
   });
 }

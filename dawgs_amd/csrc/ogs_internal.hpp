@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <memory>
 
 namespace ogs_hip {
 
@@ -81,6 +83,12 @@ struct DeltaPack {
 
 // The opaque plan behind ogs_hip_plan_t.
 struct ogs_hip_plan_s {
+  // holders: the creator, ogs_hip_plan_retain callers, the batches over the
+  // plan; the last ogs_hip_plan_destroy frees it
+  std::atomic<int> refs{1};
+  // what the plan keeps alive besides its own arrays (the CSR of a plan made
+  // by an ogs_t setup), released with the plan
+  std::shared_ptr<void> keep;
   int device = 0;
   int cus = 0;  // compute units of the device (persistent launch sizes)
   int32_t Ncols = 0, NrowsN = 0, NrowsT = 0, nnzN = 0, nnzT = 0;
@@ -143,4 +151,6 @@ struct DeviceSetup {
 // into h_ids.  Returns OGS_E_HALO_POSITIVE for a Halo group without exactly
 // one positive id, or a hipError_t.
 int device_setup(int32_t N, int64_t *h_ids, int kind, int unique, hipStream_t st, DeviceSetup *out);
+// Makes `plan` hold `keep` (device arrays it reads) until it is freed.
+inline void plan_keep(ogs_hip_plan_s *plan, std::shared_ptr<void> keep) { plan->keep = std::move(keep); }
 }

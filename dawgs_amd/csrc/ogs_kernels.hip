@@ -3346,7 +3346,13 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     else if (Ncols > 0 && 2 * (int64_t)p->nnzN >= Ncols)
       rc = build_col_rows(p->rsN, NrowsT, p->ciN, Ncols, st, &p->colRowN);
   }
-  if (!rc && !p->aliased && p->colRowT && p->maxRowN <= 1) {
+  // the N list as a column mask: GatherScatter Trans over the T list
+  if (!rc && !p->aliased && p->colRowT && p->nnzT > 0) rc = build_n_mask(p, NrowsT, scratch, st);
+  // the column -> owner map rests on what the mask's check establishes (T
+  // columns distinct, every N entry in its own T row): each column of a row
+  // then reads the row's one owner, and the owner's own rewrite of its value
+  // is idempotent
+  if (!rc && !p->aliased && p->nMask && p->maxRowN <= 1) {
     if (hipMalloc(&p->colOwner, sizeof(int32_t) * (size_t)Ncols) != hipSuccess) {
       (void)hipGetLastError();  // no room: the row-based kernels run this GatherScatter
       p->colOwner = nullptr;
@@ -3356,8 +3362,6 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
       rc = (int)hipGetLastError();
     }
   }
-  // the N list as a column mask: GatherScatter Trans over the T list
-  if (!rc && !p->aliased && p->colRowT && p->nnzT > 0) rc = build_n_mask(p, NrowsT, scratch, st);
   // T lists without empty rows (every operator that Setup builds) are cut on
   // entries alone: measured faster than merge-path blocks of the same width
   // for GatherScatter and Gather (more entries per workgroup, fewer blocks).
@@ -3414,9 +3418,15 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
   return OGS_SUCCESS;
 }
 
+int ogs_hip_plan_retain(ogs_hip_plan_t plan) {
+  if (!plan) return OGS_E_ARG;
+  plan->refs.fetch_add(1, std::memory_order_relaxed);
+  return OGS_SUCCESS;
+}
+
 int ogs_hip_plan_destroy(ogs_hip_plan_t plan) {
   if (!plan) return OGS_E_ARG;
-  free_plan(plan);
+  if (plan->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) free_plan(plan);
   return OGS_SUCCESS;
 }
 
@@ -3438,6 +3448,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->colsN = p->colRowN != nullptr;
   info->deltaE = p->dpack.w ? p->dpack.E : 0;
   info->deltaFallback = p->dpack.w ? p->dpack.fallback : 0;
+  info->ownerMap = p->colOwner != nullptr;
+  info->nMask = p->nMask != nullptr;
   info->bytes = plan_bytes(p);
   return OGS_SUCCESS;
 }
@@ -3701,6 +3713,7 @@ static int upload_tables(const BatchTables &t, hipStream_t st, BatchDevTables &o
 
 struct ogs_hip_batch_s {
   int type = 0, op = 0;
+  std::vector<ogs_hip_plan_t> plans;  // retained while the batch lives
   ogs_hip::BatchDevTables tables;
   ogs_hip::BatchConfig cfg = {0, 1, 1};
   int cus = 1;
@@ -3720,6 +3733,7 @@ static void free_batch(ogs_hip_batch_s *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->tables.release();
+  for (ogs_hip_plan_t p : b->plans) (void)ogs_hip_plan_destroy(p);
   delete b;
 }
 
@@ -3835,6 +3849,12 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (rc) {
     free_batch(b);
     return rc;
+  }
+  // the graph reads the plans' arrays: hold each plan once
+  for (int i = 0; i < n; ++i) {
+    if (std::find(b->plans.begin(), b->plans.end(), in[i].plan) != b->plans.end()) continue;
+    (void)ogs_hip_plan_retain(in[i].plan);
+    b->plans.push_back(in[i].plan);
   }
   *batch = b;
   return OGS_SUCCESS;

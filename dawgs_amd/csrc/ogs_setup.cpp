@@ -198,6 +198,12 @@ void ogsOperator_t::createPlan(const platform_t &platform) {
   const int rc = ogs_hip_plan_create(&plan_, Ncols, NrowsN, NrowsT, o_rowStartsN.ptr(), o_colIdsN.ptr(),
                                      o_rowStartsT.ptr(), o_colIdsT.ptr(), platform.stream);
   LIBP_ABORT("ogs_hip_plan_create: " << ogs_hip_error_string(rc), rc != 0, rc);
+  // the plan reads these lists: it shares their ownership, so that a batch or
+  // another holder of the plan keeps them alive past this operator
+  struct Lists {
+    deviceMemory<dlong> rsN, ciN, rsT, ciT;
+  };
+  ogs_hip::plan_keep(plan_, std::make_shared<Lists>(Lists{o_rowStartsN, o_colIdsN, o_rowStartsT, o_colIdsT}));
 }
 
 ogsOperator_t::~ogsOperator_t() {

@@ -52,6 +52,13 @@ def type_of(t, type=None) -> int:
     return type
 
 
+def _need(t, n: int, name: str) -> None:
+    """The kernels index a buffer up to the sizes the operator implies; a
+    shorter one is refused before anything is launched."""
+    if t.numel() < n:
+        raise ValueError(f"{name} holds {t.numel()} values, the operator needs {n}")
+
+
 def _dev_ptr(t) -> int:
     if t is None:
         return None
@@ -93,14 +100,25 @@ class Plan:
                                         _stream(stream)), "ogs_hip_plan_create")
         self.h = h
         self.owned = True
+        self._sizes()
 
     @classmethod
     def borrowed(cls, handle):
+        """A plan owned elsewhere (an ogs_t's), held by one more reference
+        (ogs_hip_plan_retain) for this object's lifetime: it stays usable, and
+        keeps its CSR alive, after its owner is freed."""
         p = cls.__new__(cls)
         p.h = C.c_void_p(handle)
-        p.owned = False
         p._keep = None
+        p.owned = False
+        check(lib().ogs_hip_plan_retain(p.h), "ogs_hip_plan_retain")
+        p.owned = True
+        p._sizes()
         return p
+
+    def _sizes(self) -> None:
+        i = self.info()
+        self.Ncols, self.NrowsN, self.NrowsT = i["Ncols"], i["NrowsN"], i["NrowsT"]
 
     def info(self) -> dict:
         i = _lib.PlanInfo()
@@ -108,14 +126,19 @@ class Plan:
         return {n: getattr(i, n) for n, _ in i._fields_}
 
     def GatherScatter(self, v, k, op, trans, stream=None, type=None):
+        _need(v, self.Ncols * k, "v")
         check(lib().ogs_hip_gather_scatter(self.h, type_of(v, type), op, trans, k, _dev_ptr(v), _stream(stream)),
               "ogs_hip_gather_scatter")
 
     def Gather(self, gv, v, k, op, trans, stream=None, type=None):
+        _need(v, self.Ncols * k, "v")
+        _need(gv, (self.NrowsN if trans == NoTrans else self.NrowsT) * k, "gv")
         check(lib().ogs_hip_gather(self.h, type_of(v, type), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
               "ogs_hip_gather")
 
     def Scatter(self, v, gv, k, trans, stream=None, type=None):
+        _need(v, self.Ncols * k, "v")
+        _need(gv, (self.NrowsN if trans == Trans else self.NrowsT) * k, "gv")
         check(lib().ogs_hip_scatter(self.h, type_of(v, type), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
               "ogs_hip_scatter")
 
@@ -131,7 +154,8 @@ class Batch:
     """ogs_hip_batch_t: GatherScatter Sym of many (plan, K, buffer) triples
     in one persistent launch captured in a HIP graph.  Buffers are bound at
     creation; tune=True times launch shapes on scratch copies first.  The
-    plans' owners (ogs_t objects) must outlive the batch."""
+    batch holds a reference to each plan (and through it the plan's CSR), so
+    the plans' owners may be freed first; it keeps the buffers too."""
 
     def __init__(self, items, op: int = Add, tune: bool = True, stream=None, type=None):
         items = list(items)
@@ -143,8 +167,7 @@ class Batch:
         for i, (plan, K, v) in enumerate(items):
             if type_of(v, type) != ty:
                 raise TypeError("all buffers of a batch share one dtype")
-            if v.numel() < plan.info()["Ncols"] * K:
-                raise ValueError("buffer shorter than Ncols*K")
+            _need(v, plan.Ncols * K, "buffer")
             descs[i].plan = plan.h
             descs[i].K = K
             descs[i].d_v = _dev_ptr(v)
@@ -238,16 +261,28 @@ class ogs_t:
         return out
 
     def GatherScatter(self, v, k: int, op: int, trans: int, stream=None, type=None) -> None:
+        self._ready()
+        _need(v, self.N * k, "v")
         check(lib().ogs_gather_scatter(self.h, type_of(v, type), op, trans, k, _dev_ptr(v), _stream(stream)),
               "ogs_gather_scatter")
 
     def Gather(self, gv, v, k: int, op: int, trans: int, stream=None, type=None) -> None:
+        self._ready()
+        _need(v, self.N * k, "v")
+        _need(gv, (self.NrowsN if trans == NoTrans else self.NrowsT) * k, "gv")
         check(lib().ogs_gather(self.h, type_of(v, type), op, trans, k, _dev_ptr(gv), _dev_ptr(v), _stream(stream)),
               "ogs_gather")
 
     def Scatter(self, v, gv, k: int, trans: int, stream=None, type=None) -> None:
+        self._ready()
+        _need(v, self.N * k, "v")
+        _need(gv, (self.NrowsN if trans == Trans else self.NrowsT) * k, "gv")
         check(lib().ogs_scatter(self.h, type_of(v, type), trans, k, _dev_ptr(v), _dev_ptr(gv), _stream(stream)),
               "ogs_scatter")
+
+    def _ready(self) -> None:
+        if not self.h:
+            raise RuntimeError("ogs handle is not set up")
 
 
 class halo_t:

@@ -100,6 +100,14 @@ typedef struct ogs_hip_plan_s *ogs_hip_plan_t;
 int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
                         const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
                         const int32_t *d_rowStartsT, const int32_t *d_colIdsT, void *stream);
+/* Plans are reference counted (no counterpart in the reference, whose
+ * operators are owned by shared_ptr, ogsBase.hpp:62-63): create returns one
+ * reference, retain adds one, destroy drops one and frees the plan with the
+ * last.  A batch holds a reference to each of its plans until it is
+ * destroyed.  A plan from ogs_get_plan also keeps the handle's CSR lists
+ * alive; for a plan from ogs_hip_plan_create the caller's CSR arrays must
+ * outlive the plan's last holder. */
+int ogs_hip_plan_retain(ogs_hip_plan_t plan);
 int ogs_hip_plan_destroy(ogs_hip_plan_t plan);
 
 /* ogsOperator_t::GatherScatter (ogsOperator.cpp:311-360 semantics, device
@@ -139,12 +147,16 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * entries), so that Scatter can run by columns (see ogs_hip_scatter).
  * deltaE: exception words per 512-entry range of the K=1 GatherScatter's
  * 16-bit delta pack (0 = none; the 32-bit pack serves), deltaFallback: its
- * blocks that run from the 32-bit pack.  bytes: device memory the plan owns
- * (partitions, packs, column maps, block orders), beside the caller's CSR. */
+ * blocks that run from the 32-bit pack.  ownerMap: 1 where the plan holds
+ * each column's owner (GatherScatter NoTrans by columns: a Signed operator
+ * with distinct T columns and at most one N member per row, each in its own
+ * T row), nMask: 1 where it holds the N-column mask (GatherScatter Trans over
+ * the T list).  bytes: device memory the plan owns (partitions, packs,
+ * column maps, block orders), beside the caller's CSR. */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
-  int32_t colsT, colsN, deltaE, deltaFallback;
+  int32_t colsT, colsN, deltaE, deltaFallback, ownerMap, nMask;
   int64_t bytes;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
@@ -163,7 +175,9 @@ int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
  * buffers (best of two passes each) and keeps the fastest; the buffers
  * themselves are not touched.
  * Creation allocates and synchronises; ogs_hip_batch_run only launches the
- * graph.  Concurrent runs of one batch race on its buffers. */
+ * graph.  Concurrent runs of one batch race on its buffers.  The batch
+ * retains its plans (see ogs_hip_plan_retain), so a plan's other holders
+ * may be destroyed first; the value buffers stay the caller's to keep. */
 typedef struct {
   ogs_hip_plan_t plan;
   int32_t K;
@@ -217,7 +231,8 @@ int ogs_get_info(ogs_handle_t h, ogs_info_t *info);
 int ogs_get_csr(ogs_handle_t h, int32_t *rowStartsN, int32_t *colIdsN,
                 int32_t *rowStartsT, int32_t *colIdsT,
                 int32_t *blockRowStartsN, int32_t *blockRowStartsT);
-/* The operator plan of the local operator (owned by the handle). */
+/* The operator plan of the local operator (the handle holds one reference;
+ * a caller that keeps the plan past ogs_free retains it first). */
 ogs_hip_plan_t ogs_get_plan(ogs_handle_t h);
 
 /* ogs_t::GatherScatter on a device buffer (ogs.cpp:39-123). */

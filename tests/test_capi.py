@@ -79,3 +79,41 @@ def test_make_ids_matches_oracle():
         a = make_ids(*box)
         assert np.array_equal(a, O.make_ids(*box))
         assert a.max() == mesh_counts(*box)["NgatherGlobal"]
+
+
+def test_short_buffers_refused_before_launch():
+    """The Python mirror checks buffer lengths against the operator's sizes
+    before calling into the library (a short buffer would be indexed past
+    its end on the device)."""
+    import torch
+    import dawgs_amd as D
+    from dawgs_amd.ogs import Plan
+    p = Plan.__new__(Plan)
+    p.h, p.owned, p._keep = None, False, None
+    p.Ncols, p.NrowsN, p.NrowsT = 10, 3, 4
+    with pytest.raises(ValueError, match="v holds 9"):
+        p.GatherScatter(torch.zeros(9), 1, D.Add, D.Sym)
+    with pytest.raises(ValueError, match="v holds 19"):
+        p.GatherScatter(torch.zeros(19), 2, D.Add, D.Sym)
+    with pytest.raises(ValueError, match="gv holds 3 values, the operator needs 4"):
+        p.Gather(torch.zeros(3), torch.zeros(10), 1, D.Add, D.Trans)
+    with pytest.raises(ValueError, match="gv holds 2 values, the operator needs 3"):
+        p.Gather(torch.zeros(2), torch.zeros(10), 1, D.Add, D.NoTrans)
+    with pytest.raises(ValueError, match="gv holds 5 values, the operator needs 6"):
+        p.Scatter(torch.zeros(20), torch.zeros(5), 2, D.Trans)
+    with pytest.raises(ValueError, match="gv holds 7 values, the operator needs 8"):
+        p.Scatter(torch.zeros(20), torch.zeros(7), 2, D.NoTrans)
+    g = D.ogs_t()
+    with pytest.raises(RuntimeError, match="not set up"):
+        g.GatherScatter(torch.zeros(4), 1, D.Add, D.Sym)
+    g.h = C.c_void_p(1)  # stands in for a set-up handle; never passed to the library here
+    g.N, g.NrowsN, g.NrowsT = 8, 2, 5
+    try:
+        with pytest.raises(ValueError, match="v holds 15 values, the operator needs 16"):
+            g.GatherScatter(torch.zeros(15), 2, D.Add, D.Sym)
+        with pytest.raises(ValueError, match="gv holds 4 values, the operator needs 5"):
+            g.Gather(torch.zeros(4), torch.zeros(8), 1, D.Add, D.Trans)
+        with pytest.raises(ValueError, match="gv holds 1 values, the operator needs 2"):
+            g.Scatter(torch.zeros(8), torch.zeros(1), 1, D.Trans)
+    finally:
+        g.h = None
