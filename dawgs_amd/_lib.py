@@ -52,6 +52,7 @@ class HaloInfo(C.Structure):
 SIGNATURES = {
     "ogs_hip_error_string": (C.c_char_p, [C.c_int]),
     "ogs_hip_plan_create": (C.c_int, [C.POINTER(_P), _I32, _I32, _I32, _P, _P, _P, _P, _P]),
+    "ogs_hip_plan_create_k": (C.c_int, [C.POINTER(_P), _I32, _I32, _I32, _P, _P, _P, _P, C.c_int, _P]),
     "ogs_hip_plan_retain": (C.c_int, [_P]),
     "ogs_hip_plan_destroy": (C.c_int, [_P]),
     "ogs_hip_gather_scatter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P]),
@@ -64,6 +65,7 @@ SIGNATURES = {
     "ogs_hip_batch_destroy": (C.c_int, [_P]),
     "ogs_hip_batch_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
     "ogs_setup": (C.c_int, [C.POINTER(_P), _I32, _P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ogs_setup_k": (C.c_int, [C.POINTER(_P), _I32, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ogs_free": (C.c_int, [_P]),
     "ogs_get_info": (C.c_int, [_P, C.POINTER(OgsInfo)]),
     "ogs_get_csr": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),

@@ -226,12 +226,19 @@ bool bad_trans(int t) { return t < OGS_SYM || t > OGS_TRANS; }
 extern "C" {
 
 int ogs_setup(ogs_handle_t *h, int32_t N, int64_t *ids, int kind, int method, int unique, int verbose) {
+  return ogs_setup_k(h, N, ids, kind, method, unique, verbose, OGS_PLAN_ALL_K);
+}
+
+int ogs_setup_k(ogs_handle_t *h, int32_t N, int64_t *ids, int kind, int method, int unique, int verbose,
+                int fields) {
   if (!h) return OGS_E_ARG;
   *h = nullptr;
+  if (fields != OGS_PLAN_ALL_K && fields != OGS_PLAN_K1) return OGS_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return OGS_E_NO_DEVICE;
   ogs_handle_s *o = new (std::nothrow) ogs_handle_s();
   if (!o) return (int)hipErrorOutOfMemory;
+  o->ogs.planFields = fields;
   const int rc = guarded([&] {
     platform_t platform;
     o->ogs.Setup(N, ids, (ogs::Kind)kind, (ogs::Method)method, unique != 0, verbose != 0, platform);

@@ -183,8 +183,8 @@ class ogsOperator_t {
   ogsOperator_t &operator=(const ogsOperator_t &) = delete;
 
   void setupRowBlocks();                 // ogsOperator.cpp:447-512
-  void upload(const platform_t &platform);  // device copies + plan
-  void createPlan(const platform_t &platform);  // plan over the device lists
+  void upload(const platform_t &platform, int fields = OGS_PLAN_ALL_K);  // device copies + plan
+  void createPlan(const platform_t &platform, int fields = OGS_PLAN_ALL_K);  // plan over the device lists
   // Host colIds, copied from the device on first use when the setup built
   // the lists there (only the CSR query reads them).
   const memory<dlong> &hostColIdsT() const;
@@ -215,6 +215,9 @@ class ogs_t {
   Kind kind = Unsigned;
   bool unique = false;
   bool gather_defined = false;
+  // field counts the local operator's plan prepares for (OGS_PLAN_ALL_K or
+  // OGS_PLAN_K1), read by Setup; no counterpart in the reference
+  int planFields = OGS_PLAN_ALL_K;
 
   void Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kind _kind, const Method method,
              const bool _unique, const bool verbose, platform_t &_platform);

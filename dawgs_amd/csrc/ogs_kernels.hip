@@ -1118,6 +1118,34 @@ __device__ __forceinline__ uint32_t delta_of(uint32_t code, const uint4 &tb) {
   return (code & 8u) ? hi : lo;
 }
 
+// The columns of a wave's 8 slots from its 16-bit words (two per dword of
+// w2) and its exception word ex (lane l holds the range's exception l): per
+// slot the deltas (an exception by its rank among the range's escapes,
+// through a lane permute), an inclusive scan, the running carry.  Columns
+// are clamped to [0, ncols), so any word content stays in bounds.
+__device__ __forceinline__ void delta_columns(const uint32_t (&w2)[4], uint32_t ex, const uint4 &tb, int ncols,
+                                              uint32_t (&col)[8]) {
+  uint32_t carry = 0u;
+  int run = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t code = (w2[j >> 1] >> ((j & 1) * 16 + 12)) & 15u;
+    uint32_t d = delta_of(code, tb);
+    const bool esc = code == kDEscape;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (mk) {
+      const int rank = (run + lanes_below(mk)) & 63;
+      const uint32_t e = (uint32_t)__builtin_amdgcn_ds_bpermute(rank * 4, (int)ex);
+      d = esc ? e : d;
+      run += __builtin_popcountll(mk);
+    }
+    const uint32_t sc = wave_scan_incl(d);
+    const uint32_t c = carry + sc;
+    col[j] = c < (uint32_t)ncols ? c : (uint32_t)(ncols - 1);
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)sc, 63);
+  }
+}
+
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict__ part,
                                                          const int32_t *__restrict__ starts,
@@ -1154,29 +1182,9 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict_
     return;
   }
   const int g0 = uniform(p0.y), n = uniform(p1.y) - g0;
-  // columns: per slot, the deltas (exceptions by rank among the range's
-  // escapes, through a lane permute), an inclusive scan, the running carry
   const uint32_t w2[4] = {wd.x, wd.y, wd.z, wd.w};
   uint32_t col[V::SL];
-  uint32_t carry = 0u;
-  int run = 0;
-#pragma unroll
-  for (int j = 0; j < V::SL; ++j) {
-    const uint32_t code = (w2[j >> 1] >> ((j & 1) * 16 + 12)) & 15u;
-    uint32_t d = delta_of(code, tb);
-    const bool esc = code == kDEscape;
-    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
-    if (mk) {
-      const int rank = (run + lanes_below(mk)) & 63;
-      const uint32_t e = (uint32_t)__builtin_amdgcn_ds_bpermute(rank * 4, (int)ex);
-      d = esc ? e : d;
-      run += __builtin_popcountll(mk);
-    }
-    const uint32_t sc = wave_scan_incl(d);
-    const uint32_t c = carry + sc;
-    col[j] = c < (uint32_t)ncols ? c : (uint32_t)(ncols - 1);  // stays in bounds whatever the words hold
-    carry += (uint32_t)__builtin_amdgcn_readlane((int)sc, 63);
-  }
+  delta_columns(w2, ex, tb, ncols, col);
   T x[V::SL];
 #pragma unroll
   for (int j = 0; j < V::SL; ++j) x[j] = q[col[j]];
@@ -1292,56 +1300,21 @@ __device__ __forceinline__ void stage_heads(const VecStage<SL> &s, const VecMeta
   }
 }
 
-// Gather of a T list without empty rows, KT interleaved fields (compile-time
-// shape): the GatherScatter body up to the row reduction (values gathered
-// through the column-sorted pack and staged at their CSR positions, rows
-// found from the head bits, so no row starts are read), each (row, field
-// chunk)'s result stored at its row -- the number of row heads at or before
-// its head's position -- instead of scattered back.
+// The rows of a Gather block whose n entries' values are staged at their CSR
+// positions with their head bytes (the staging barrier passed): the head
+// bit mask and its per-word prefix counts, then one lane per (row head,
+// field chunk) folds the row in CSR order and stores the result at its row,
+// r0 + the number of row heads at or before its head.
 template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
-                                                       const int32_t *__restrict__ starts,
-                                                       const int32_t *__restrict__ ids,
-                                                       const uint32_t *__restrict__ pk,
-                                                       const int4 *__restrict__ base, int stride,
-                                                       const T *__restrict__ q, T *__restrict__ gq,
-                                                       const int *__restrict__ order) {
+__device__ __forceinline__ void gather_rows(int n, int r0, const typename Lds<T>::L *__restrict__ val,
+                                            const uint8_t *__restrict__ head, uint32_t *__restrict__ hm,
+                                            int32_t *__restrict__ pc, T *__restrict__ gq) {
   using V = VecShape<T, KT>;
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
-  constexpr int SL = V::SL, VW = V::VW, C = V::C;
-  __shared__ L val[vec_cap<L>()];
-  __shared__ uint32_t head4[kCap / 4];
-  __shared__ uint32_t hm[kCap / 32 + 1];
-  __shared__ int32_t pc[kCap / 32];
-  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
-  const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
-  VecStage<SL> s;
-  VecMeta m;
-  T x[SL][VW];
-  const int xb = xcd_block(blockIdx.x, gridDim.x);
-  const int b = order ? order[xb] : xb;
-  vec_meta(src, b, m);
-  vec_index<T, KT>(s, src, b, m);
-  vec_gather<T, KT>(s, x);
-  const int n = s.n;
-  if (n == 0) return;
+  constexpr int VW = V::VW, C = V::C;
   const int ni = n * C;
   const int tid = threadIdx.x;
-  const int r0 = uniform(m.p0.x);
-  const bool packed = s.bb.x >= 0;
-#pragma unroll
-  for (int j = 0; j < SL; ++j) {
-    const int it = tid + j * kBlock;
-    if (it < ni) {
-      const int c = it % C;
-      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
-#pragma unroll
-      for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = Lds<T>::in(x[j][v]);
-    }
-  }
-  stage_heads(s, m, C, starts, head);
-  __syncthreads();
   head_mask(head, n, hm);
   __syncthreads();
   head_prefix(hm, n, pc);
@@ -1363,6 +1336,120 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
 #pragma unroll
     for (int v = 0; v < VW; ++v) dst[v] = Lds<T>::out(R::fin(red[v]));
   }
+}
+
+template <typename T, int OP, int KT>
+__device__ __forceinline__ void gather_vec_block(const VecSrc &src, int b, const int32_t *__restrict__ starts,
+                                                 T *__restrict__ gq, typename Lds<T>::L *__restrict__ val,
+                                                 uint8_t *__restrict__ head, uint32_t *__restrict__ hm,
+                                                 int32_t *__restrict__ pc) {
+  using V = VecShape<T, KT>;
+  constexpr int SL = V::SL, VW = V::VW, C = V::C;
+  VecStage<SL> s;
+  VecMeta m;
+  T x[SL][VW];
+  vec_meta(src, b, m);
+  vec_index<T, KT>(s, src, b, m);
+  vec_gather<T, KT>(s, x);
+  const int n = s.n;
+  if (n == 0) return;
+  const int ni = n * C;
+  const int tid = threadIdx.x;
+  const int r0 = uniform(m.p0.x);
+  const bool packed = s.bb.x >= 0;
+#pragma unroll
+  for (int j = 0; j < SL; ++j) {
+    const int it = tid + j * kBlock;
+    if (it < ni) {
+      const int c = it % C;
+      const int ps = packed ? (int)(s.w[j] & kPackPosMask) : it / C;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) val[ps * KT + c * VW + v] = Lds<T>::in(x[j][v]);
+    }
+  }
+  stage_heads(s, m, C, starts, head);
+  __syncthreads();
+  gather_rows<T, OP, KT>(n, r0, val, head, hm, pc, gq);
+}
+
+// Gather of a T list without empty rows, KT interleaved fields (compile-time
+// shape): the GatherScatter body up to the row reduction (values gathered
+// through the column-sorted pack and staged at their CSR positions, rows
+// found from the head bits, so no row starts are read), each (row, field
+// chunk)'s result stored at its row -- the number of row heads at or before
+// its head's position -- instead of scattered back.
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ part,
+                                                       const int32_t *__restrict__ starts,
+                                                       const int32_t *__restrict__ ids,
+                                                       const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       const T *__restrict__ q, T *__restrict__ gq,
+                                                       const int *__restrict__ order) {
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
+  const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  gather_vec_block<T, OP, KT>(src, b, starts, gq, val, reinterpret_cast<uint8_t *>(head4), hm, pc);
+}
+
+// Gather K = 1 over partition 0 from the delta pack (see k_gs_sym_delta):
+// columns decoded per wave, values staged at their CSR positions, rows
+// reduced and stored as in k_gather_vec; fallback blocks run k_gather_vec's
+// body from the 32-bit pack.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gather_delta(const int4 *__restrict__ part,
+                                                         const int32_t *__restrict__ starts,
+                                                         const int32_t *__restrict__ ids,
+                                                         const uint32_t *__restrict__ pk,
+                                                         const int4 *__restrict__ base, int stride,
+                                                         const u32x4 *__restrict__ dw,
+                                                         const uint32_t *__restrict__ dx, int E, uint4 tb,
+                                                         int ncols, const T *__restrict__ q, T *__restrict__ gq,
+                                                         const int *__restrict__ order) {
+  using V = VecShape<T, 1>;
+  static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
+  using L = typename Lds<T>::L;
+  __shared__ L val[vec_cap<L>()];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
+  uint8_t *head = reinterpret_cast<uint8_t *>(head4);
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int4 p0 = part[b], p1 = part[b + 1];
+  const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
+  const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + wv) * E + lane) : 0u;
+  if ((uint32_t)uniform((int)ex) == kDFallback) {
+    const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
+    gather_vec_block<T, OP, 1>(src, b, starts, gq, val, head, hm, pc);
+    return;
+  }
+  const int r0 = uniform(p0.x);
+  const int n = uniform(p1.y) - uniform(p0.y);
+  const uint32_t w2[4] = {wd.x, wd.y, wd.z, wd.w};
+  uint32_t col[V::SL];
+  delta_columns(w2, ex, tb, ncols, col);
+  T x[V::SL];
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) x[j] = q[col[j]];
+  const int i0 = wv * kDWave + lane;
+#pragma unroll
+  for (int j = 0; j < V::SL; ++j) {
+    if (i0 + j * 64 < n) {
+      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
+      const int ps = (int)(wj & kPackPosMask);
+      val[ps] = Lds<T>::in(x[j]);
+      head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
+    }
+  }
+  __syncthreads();
+  gather_rows<T, OP, 1>(n, r0, val, head, hm, pc, gq);
 }
 
 // Scatter of a T list without empty rows, KT interleaved fields: the block's
@@ -2689,6 +2776,33 @@ static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, cons
   return (int)hipGetLastError();
 }
 
+// Gather K = 1 of the T list over partition 0 from the delta pack.
+template <typename T, int OP>
+static void launch_gather_delta_op(const Plan *p, const T *v, T *gv, hipStream_t st) {
+  const Partition &pt = p->partT[0];
+  const Pack &k = p->packT[0];
+  const DeltaPack &d = p->dpack;
+  hipLaunchKernelGGL((k_gather_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT, k.pk,
+                     k.base, k.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
+                     make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), p->Ncols, v, gv,
+                     (pt.tiled & kTiledGather) ? pt.order : nullptr);
+}
+
+template <typename T>
+static int launch_gather_delta(int op, const Plan *p, const T *v, T *gv, hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_gather_delta_op<T, OGS_ADD>(p, v, gv, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_gather_delta_op<T, OGS_MUL>(p, v, gv, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_gather_delta_op<T, OGS_MAX>(p, v, gv, st); break;
+    case OGS_MIN: launch_gather_delta_op<T, OGS_MIN>(p, v, gv, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
 template <typename T>
 static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_gv, const void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:71-79
@@ -2700,6 +2814,8 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
   // the shape, narrowed to fill the device on small lists
   if (vec_k(K) && (!useN || p->aliased) && !p->partT[0].rowsInKey && p->nnzT > 0) {
     const Partition *vp = vec_launch_partition<T>(p, K);
+    if (vp && K == 1 && vp == &p->partT[0] && p->dpack.w)
+      return launch_gather_delta<T>(op, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv), st);
     if (vp) {
       const ListView G = list_view(p, false, vp, false);
       const int *order = (K == 1 && (vp->tiled & kTiledGather)) ? vp->order : nullptr;
@@ -2978,15 +3094,27 @@ static int tile_schedule(Plan *p, hipStream_t st) {
   if (!rc && DT) {
     rc = upload(pt, DT, &oT);
     const Pack &k = p->packT[0];
+    const DeltaPack &dp = p->dpack;
+    const u32x4 *dw = reinterpret_cast<const u32x4 *>(dp.w);
+    const uint4 tb = make_uint4(dp.tbl[0], dp.tbl[1], dp.tbl[2], dp.tbl[3]);
+    // the kernels the K = 1 launches will run: over the delta pack when the plan has one
     if (!rc)
       rc = order_pays([&](const int *o) {
-        hipLaunchKernelGGL((k_gs_sym_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
-                           p->ciT, k.pk, k.base, k.stride, 0, vals, o);
+        if (dp.w)
+          hipLaunchKernelGGL((k_gs_sym_delta<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
+                             p->ciT, k.pk, k.base, k.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, o);
+        else
+          hipLaunchKernelGGL((k_gs_sym_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d,
+                             p->rsT, p->ciT, k.pk, k.base, k.stride, 0, vals, o);
       }, oT, st, e0, e1, &gs);
     if (!rc)
       rc = order_pays([&](const int *o) {
-        hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
-                           p->ciT, k.pk, k.base, k.stride, vals, rows, o);
+        if (dp.w)
+          hipLaunchKernelGGL((k_gather_delta<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
+                             p->ciT, k.pk, k.base, k.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, rows, o);
+        else
+          hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d,
+                             p->rsT, p->ciT, k.pk, k.base, k.stride, vals, rows, o);
       }, oT, st, e0, e1, &ga);
   }
   if (!rc && DS) {
@@ -3291,9 +3419,18 @@ const char *ogs_hip_error_string(int code) {
 int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
                         const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
                         const int32_t *d_rowStartsT, const int32_t *d_colIdsT, void *stream) {
+  return ogs_hip_plan_create_k(plan, Ncols, NrowsN, NrowsT, d_rowStartsN, d_colIdsN, d_rowStartsT, d_colIdsT,
+                               OGS_PLAN_ALL_K, stream);
+}
+
+int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
+                          const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
+                          const int32_t *d_rowStartsT, const int32_t *d_colIdsT, int fields, void *stream) {
   if (!plan) return OGS_E_ARG;
   *plan = nullptr;
   if (Ncols < 0 || NrowsN < 0 || NrowsT < 0 || NrowsN > NrowsT) return OGS_E_ARG;
+  if (fields != OGS_PLAN_ALL_K && fields != OGS_PLAN_K1) return OGS_E_ARG;
+  const bool k1 = fields == OGS_PLAN_K1;
   if (!d_rowStartsN || !d_rowStartsT) return OGS_E_ARG;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Plan *p = new (std::nothrow) Plan();
@@ -3386,10 +3523,16 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
     }
     p->nparts = j + 1;
   }
-  // column-sorted packs of the K=1 partitions
+  // column-sorted packs of the K=1 partitions (a K = 1 plan: only those its
+  // launches read -- the partition the one-list kernels run on, and for a
+  // Signed operator the half-width one of the two-list kernel)
+  const int jK1 = p->nparts > 0 ? (int)(fill_device(p, &p->partT[0]) - p->partT) : 0;
+  const int jHalf = p->nparts > 1 ? (int)(fill_device(p, &p->partT[1]) - p->partT) : -1;
   if (!rc && p->nparts > 0) {
-    for (int j = 0; j < kPackParts && j < p->nparts && !rc; ++j)
+    for (int j = 0; j < kPackParts && j < p->nparts && !rc; ++j) {
+      if (k1 && j != jK1 && (p->aliased || j != jHalf)) continue;
       rc = build_pack(p->partT[j], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->packT[j]);
+    }
     if (!rc && p->aliased) {
       p->packNT = p->packT[0];
       p->packNN = p->packT[0];
@@ -3403,7 +3546,7 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
       else p->packS = p->packT[0];
     }
     // three-field partitions and packs
-    for (int i = 0; i < 2 && !rc && cutTOnEntries; ++i) {
+    for (int i = 0; i < 2 && !rc && cutTOnEntries && !k1; ++i) {
       const int W = kCap3[i] - (lmax > 0 ? lmax - 1 : 0);
       if (W < 1) continue;
       rc = build_partition(p, false, false, NrowsT, W, scratch, st, &p->part3[i]);
@@ -3413,6 +3556,15 @@ int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int
   if (!rc && cutTOnEntries) rc = build_delta_pack(p, scratch, st);
   (void)hipFree(scratch);
   if (!rc && cutTOnEntries) rc = tile_schedule(p, st);
+  // a K = 1 Unsigned plan whose K = 1 kernels all read the delta pack (no
+  // fallback blocks; Scatter by columns) keeps no 32-bit pack of partition 0
+  if (!rc && k1 && p->aliased && jK1 == 0 && p->dpack.w && p->dpack.fallback == 0 && p->colRowT &&
+      p->packT[0].pk) {
+    const uint32_t *gone = p->packT[0].pk;
+    for (Pack *a : {&p->packNT, &p->packNN, &p->packS})
+      if (a->pk == gone) *a = Pack();
+    free_pack(p->packT[0]);
+  }
   if (rc) { free_plan(p); return rc; }
   *plan = p;
   return OGS_SUCCESS;

@@ -109,8 +109,8 @@ void ogs_t::Setup(const dlong _N, hlong *ids, const Kind _kind, const Method met
 
   L.setupRowBlocks();             // ogsSetup.cpp:681, 774
   gatherHalo->setupRowBlocks();   // ogsSetup.cpp:682, 775
-  if (L.o_rowStartsT.ptr()) L.createPlan(platform);
-  else L.upload(platform);
+  if (L.o_rowStartsT.ptr()) L.createPlan(platform, planFields);
+  else L.upload(platform, planFields);
   gatherHalo->upload(platform);
 
   if (verbose) {
@@ -180,7 +180,7 @@ const memory<dlong> &ogsOperator_t::hostColIdsN() const {
   return colIdsN;
 }
 
-void ogsOperator_t::upload(const platform_t &platform) {
+void ogsOperator_t::upload(const platform_t &platform, int fields) {
   o_rowStartsT = platform.malloc<dlong>((size_t)NrowsT + 1, rowStartsT.ptr());
   o_colIdsT = platform.malloc<dlong>((size_t)nnzT, colIdsT.ptr());
   if (aliased) {
@@ -190,13 +190,13 @@ void ogsOperator_t::upload(const platform_t &platform) {
     o_rowStartsN = platform.malloc<dlong>((size_t)NrowsT + 1, rowStartsN.ptr());
     o_colIdsN = platform.malloc<dlong>((size_t)nnzN, colIdsN.ptr());
   }
-  createPlan(platform);
+  createPlan(platform, fields);
 }
 
-void ogsOperator_t::createPlan(const platform_t &platform) {
+void ogsOperator_t::createPlan(const platform_t &platform, int fields) {
   if (plan_) { ogs_hip_plan_destroy(plan_); plan_ = nullptr; }
-  const int rc = ogs_hip_plan_create(&plan_, Ncols, NrowsN, NrowsT, o_rowStartsN.ptr(), o_colIdsN.ptr(),
-                                     o_rowStartsT.ptr(), o_colIdsT.ptr(), platform.stream);
+  const int rc = ogs_hip_plan_create_k(&plan_, Ncols, NrowsN, NrowsT, o_rowStartsN.ptr(), o_colIdsN.ptr(),
+                                       o_rowStartsT.ptr(), o_colIdsT.ptr(), fields, platform.stream);
   LIBP_ABORT("ogs_hip_plan_create: " << ogs_hip_error_string(rc), rc != 0, rc);
   // the plan reads these lists: it shares their ownership, so that a batch or
   // another holder of the plan keeps them alive past this operator

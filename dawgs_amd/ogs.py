@@ -25,6 +25,8 @@ Add, Mul, Max, Min = 0, 1, 2, 3
 Sym, NoTrans, Trans = 0, 1, 2
 Auto, Pairwise, CrystalRouter, AllToAll = 0, 1, 2, 3
 Unsigned, Signed, Halo = 0, 1, 2
+# field counts a plan prepares for (ogs_hip_plan_create_k): every K, or K = 1 only
+PlanAllK, PlanK1 = 0, 1
 
 
 _TYPES = None  # torch dtype -> type code, built on first use (torch imported lazily)
@@ -91,13 +93,14 @@ def _stream(stream) -> int:
 class Plan:
     """ogs_hip_plan_t over a caller-owned device CSR pair (operator layer)."""
 
-    def __init__(self, Ncols, NrowsN, NrowsT, rowStartsN, colIdsN, rowStartsT, colIdsT, stream=None):
+    def __init__(self, Ncols, NrowsN, NrowsT, rowStartsN, colIdsN, rowStartsT, colIdsT, stream=None,
+                 fields: int = PlanAllK):
         self._keep = (rowStartsN, colIdsN, rowStartsT, colIdsT)
         h = C.c_void_p()
-        check(lib().ogs_hip_plan_create(C.byref(h), Ncols, NrowsN, NrowsT,
-                                        _dev_ptr(rowStartsN), _dev_ptr(colIdsN) if colIdsN.numel() else None,
-                                        _dev_ptr(rowStartsT), _dev_ptr(colIdsT) if colIdsT.numel() else None,
-                                        _stream(stream)), "ogs_hip_plan_create")
+        check(lib().ogs_hip_plan_create_k(C.byref(h), Ncols, NrowsN, NrowsT,
+                                          _dev_ptr(rowStartsN), _dev_ptr(colIdsN) if colIdsN.numel() else None,
+                                          _dev_ptr(rowStartsT), _dev_ptr(colIdsT) if colIdsT.numel() else None,
+                                          fields, _stream(stream)), "ogs_hip_plan_create_k")
         self.h = h
         self.owned = True
         self._sizes()
@@ -206,17 +209,19 @@ class ogs_t:
         self.h = None
 
     def Setup(self, N: int, ids: np.ndarray, kind: int = Unsigned, method: int = Auto,
-              unique: bool = False, verbose: bool = False) -> None:
+              unique: bool = False, verbose: bool = False, fields: int = PlanAllK) -> None:
         """ogs_t::Setup (ogsSetup.cpp:44-190).  `ids` (int64) is rewritten
-        in place with the chosen signs when unique=True (ogsSetup.cpp:139-140)."""
+        in place with the chosen signs when unique=True (ogsSetup.cpp:139-140).
+        `fields` (no counterpart in the reference): PlanK1 builds the operator
+        plan for K = 1 only (other K still run, more slowly)."""
         if not (isinstance(ids, np.ndarray) and ids.dtype == np.int64 and ids.flags.c_contiguous):
             raise TypeError("ids must be a contiguous numpy int64 array")
         if N > len(ids):
             raise ValueError("N exceeds len(ids)")
         self.Free()
         h = C.c_void_p()
-        check(lib().ogs_setup(C.byref(h), N, ids.ctypes.data_as(C.c_void_p), kind, method,
-                              int(unique), int(verbose)), "ogs_setup")
+        check(lib().ogs_setup_k(C.byref(h), N, ids.ctypes.data_as(C.c_void_p), kind, method,
+                                int(unique), int(verbose), fields), "ogs_setup_k")
         self.h = h
         i = _lib.OgsInfo()
         check(lib().ogs_get_info(self.h, C.byref(i)), "ogs_get_info")

@@ -100,6 +100,19 @@ typedef struct ogs_hip_plan_s *ogs_hip_plan_t;
 int ogs_hip_plan_create(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
                         const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
                         const int32_t *d_rowStartsT, const int32_t *d_colIdsT, void *stream);
+/* ogs_hip_plan_create with a choice of the field counts the plan prepares
+ * for (no counterpart in the reference; ogs_hip_plan_create is
+ * OGS_PLAN_ALL_K).  OGS_PLAN_K1 keeps only what the K = 1 launches read: the
+ * row partitions, the column maps, the pack of the partition the K = 1
+ * kernels run on (a Signed operator: and of the two-list kernel's), the
+ * 16-bit delta pack, and no 32-bit pack where the delta pack serves every
+ * K = 1 kernel of an Unsigned operator.  Any other K still runs, from the
+ * CSR (without packs: slower, same results).  48^3 p=7 Unsigned: 0.35 GB
+ * instead of 2.06 GB. */
+enum { OGS_PLAN_ALL_K = 0, OGS_PLAN_K1 = 1 };
+int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, int32_t NrowsT,
+                          const int32_t *d_rowStartsN, const int32_t *d_colIdsN,
+                          const int32_t *d_rowStartsT, const int32_t *d_colIdsT, int fields, void *stream);
 /* Plans are reference counted (no counterpart in the reference, whose
  * operators are owned by shared_ptr, ogsBase.hpp:62-63): create returns one
  * reference, retain adds one, destroy drops one and frees the plan with the
@@ -214,6 +227,10 @@ typedef struct ogs_handle_s *ogs_handle_t;
  * operator plan. */
 int ogs_setup(ogs_handle_t *h, int32_t N, int64_t *ids, int kind, int method,
               int unique, int verbose);
+/* ogs_setup whose operator plan prepares for `fields` (OGS_PLAN_ALL_K, what
+ * ogs_setup does, or OGS_PLAN_K1; see ogs_hip_plan_create_k). */
+int ogs_setup_k(ogs_handle_t *h, int32_t N, int64_t *ids, int kind, int method,
+                int unique, int verbose, int fields);
 int ogs_free(ogs_handle_t h);
 
 typedef struct {

@@ -168,3 +168,94 @@ def test_delta_dropped_when_mostly_fallback(torch, D):
     g.GatherScatter(d, 1, D.Add, D.Sym)
     torch.cuda.synchronize()
     assert same(d.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("box", BOXES, ids=lambda b: "x".join(map(str, b)))
+@pytest.mark.parametrize("ty", list(TYPES))
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_delta_gather(torch, D, box, ty, op):
+    """Gather K = 1 (Trans, and NoTrans of the Unsigned operator) over the
+    delta pack, each row's result at its row."""
+    g, ref, N = unsigned_mesh(D, box)
+    dt = TYPES[ty]
+    rng = np.random.default_rng(hash((box, ty, op, "g")) % 2 ** 32)
+    v = values(rng, dt, N)
+    dv = torch.from_numpy(v.copy()).cuda()
+    for trans in (O.TRANS, O.NOTRANS):
+        want = ref.gather(v, 1, op, trans)
+        gq = torch.zeros(len(want), dtype=dv.dtype, device="cuda")
+        g.plan.Gather(gq, dv, 1, op, trans)
+        torch.cuda.synchronize()
+        assert same(gq.cpu().numpy(), want), (box, np.dtype(dt).name, op, trans)
+
+
+def test_delta_gather_fallback_blocks(torch, D):
+    ids = shuffled((36, 32, 32, 3), 0.08, 3)
+    g, _ = setup(D, ids)
+    assert g.plan.info()["deltaFallback"] > 0
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids))
+    v = values(np.random.default_rng(6), np.float64, len(ids))
+    want = ref.gather(v, 1, O.ADD, O.TRANS)
+    gq = torch.zeros(len(want), dtype=torch.float64, device="cuda")
+    g.plan.Gather(gq, torch.from_numpy(v).cuda(), 1, D.Add, D.Trans)
+    torch.cuda.synchronize()
+    assert same(gq.cpu().numpy(), want)
+
+
+def test_k1_plan_48cubed_p7(torch, D):
+    """A plan prepared for K = 1 only (OGS_PLAN_K1) at the headline size: at
+    most 0.5 GB of plan memory beside the CSR (the full plan holds ~2 GB),
+    GatherScatter / Gather / Scatter at K = 1 bit-exact, and K = 3 still
+    runs (from the CSR) bit-exact."""
+    ids = O.make_ids(48, 48, 48, 7)
+    N = len(ids)
+    g = D.ogs_t()
+    g.Setup(N, ids.copy(), D.Unsigned, D.Auto, False, False, fields=D.PlanK1)
+    info = g.plan.info()
+    assert info["bytes"] <= 500_000_000, info
+    assert info["deltaE"] > 0 and info["deltaFallback"] == 0 and info["packedK1"] == 0, info
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=N)
+    rng = np.random.default_rng(71)
+    v = rng.uniform(-1, 1, N)
+    d = torch.from_numpy(v.copy()).cuda()
+    g.GatherScatter(d, 1, D.Add, D.Sym)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), ref.gather_scatter(v.copy(), 1, O.ADD, O.SYM))
+    wg = ref.gather(v, 1, O.MAX, O.TRANS)
+    gq = torch.zeros(len(wg), dtype=torch.float64, device="cuda")
+    g.plan.Gather(gq, torch.from_numpy(v).cuda(), 1, D.Max, D.Trans)
+    torch.cuda.synchronize()
+    assert same(gq.cpu().numpy(), wg)
+    base = rng.uniform(-1, 1, N)
+    d = torch.from_numpy(base.copy()).cuda()
+    g.plan.Scatter(d, gq, 1, D.NoTrans)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), ref.scatter(base.copy(), wg, 1, O.NOTRANS))
+    del gq, d
+    x3 = rng.uniform(-1, 1, 3 * N)
+    d3 = torch.from_numpy(x3.copy()).cuda()
+    g.GatherScatter(d3, 3, D.Add, D.Sym)
+    torch.cuda.synchronize()
+    assert same(d3.cpu().numpy(), ref.gather_scatter(x3.copy(), 3, O.ADD, O.SYM))
+
+
+@pytest.mark.parametrize("unique,flag", [(True, 0.0), (False, 0.3)])
+def test_k1_plan_signed(torch, D, unique, flag):
+    """OGS_PLAN_K1 on Signed operators keeps the packs the two-list kernels
+    read; every K = 1 call stays bit-exact."""
+    rng = np.random.default_rng(21)
+    ids = O.make_ids(36, 32, 32, 3).copy()
+    if flag:
+        neg = rng.random(len(ids)) < flag
+        ids[neg] = -ids[neg]
+    o = O.OracleOgs(ids, O.SIGNED, unique)
+    g = D.ogs_t()
+    g.Setup(len(ids), ids.copy(), D.Signed, D.Auto, unique, False, fields=D.PlanK1)
+    v = rng.uniform(-1, 1, len(ids))
+    for trans in (O.SYM, O.NOTRANS, O.TRANS):
+        d = torch.from_numpy(v.copy()).cuda()
+        g.GatherScatter(d, 1, D.Add, trans)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), o.op_gather_scatter(v.copy(), 1, O.ADD, trans)), trans
