@@ -71,6 +71,7 @@ SIGNATURES = {
     "ogs_get_csr": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "ogs_get_plan": (_P, [_P]),
     "ogs_setup_global_to_local": (C.c_int, [_P, _P]),
+    "ogs_setup_global_to_local_noalloc": (C.c_int, [_P, _P]),
     "ogs_gather_scatter": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P]),
     "ogs_gather": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _I32, _P, _P, _P]),
     "ogs_scatter": (C.c_int, [_P, C.c_int, C.c_int, _I32, _P, _P, _P]),

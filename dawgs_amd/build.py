@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libogs_hip.so")
 
-SOURCES = ["ogs_kernels.hip", "ogs_setup_dev.hip", "ogs_setup.cpp", "ogs.cpp", "ogs_halo.cpp"]
+SOURCES = ["ogs_kernels.hip", "ogs_setup_dev.hip", "ogs_setup.cpp", "ogs.cpp", "ogs_halo.cpp", "ogs_noalloc.hip"]
 
 ARCH = os.environ.get("OGS_OFFLOAD_ARCH", "gfx950")
 FLAGS = [

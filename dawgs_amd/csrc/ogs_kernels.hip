@@ -3412,6 +3412,7 @@ const char *ogs_hip_error_string(int code) {
     case OGS_E_NOT_SUPPORTED: return "type/op combination not supported";
     case OGS_E_CSR: return "CSR arrays inconsistent with the given sizes";
     case OGS_E_NO_DEVICE: return "no HIP device";
+    case OGS_E_TOO_LARGE: return "input exceeds the fixed scratch of the no-allocation entry point";
     default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
   }
 }

@@ -76,7 +76,8 @@ enum {
   OGS_E_HALO_POSITIVE = -5,  /* Halo group without exactly one positive id (ogsSetup.cpp:286-288) */
   OGS_E_NOT_SUPPORTED = -6,  /* type/op combination not offered (e.g. bf16 Mul) */
   OGS_E_CSR = -7,            /* CSR arrays inconsistent with the sizes given */
-  OGS_E_NO_DEVICE = -8       /* no HIP device visible */
+  OGS_E_NO_DEVICE = -8,      /* no HIP device visible */
+  OGS_E_TOO_LARGE = -9       /* input beyond the fixed scratch of a no-allocation entry point */
 };
 
 /* Longest row the setup accepts (ogsOperator.hpp:123 gatherNodesPerBlock). */
@@ -257,6 +258,15 @@ ogs_hip_plan_t ogs_get_plan(ogs_handle_t h);
  * host array of N entries) receives each node's gathered index (its row),
  * -1 for nodes with id 0.  Synchronises. */
 int ogs_setup_global_to_local(ogs_handle_t h, int32_t *GlobalToLocal);
+/* ogs_setup_global_to_local without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The map is staged in
+ * a fixed per-GPU scratch array of 2^22 entries (16 MiB, held by each GPU the
+ * library runs on until the process ends): N above 2^22 returns
+ * OGS_E_TOO_LARGE.  Runs on the handle's stream and synchronises it.  Every
+ * call on one GPU shares that GPU's scratch, so calls that overlap on one GPU
+ * race. */
+int ogs_setup_global_to_local_noalloc(ogs_handle_t h, int32_t *GlobalToLocal);
 
 int ogs_gather_scatter(ogs_handle_t h, int type, int op, int trans, int32_t K,
                        void *d_v, void *stream);
