@@ -18,8 +18,10 @@ on the same mesh, against the plain fp64 method, with errors against the fp64
 oracle) and `batched` (48 meshes in one graph launch against one launch per
 mesh and field); --no-extras skips them.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 100] [--nx 48] [--p 7] [--no-cpu] [--copy-probe]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 100] [--nx 48 [--ny NY] [--nz NZ]] [--p 7] [--no-cpu]
+                    [--copy-probe] [--idle-probe]
     python bench.py --sweep [DEGREES] [--k K] | --batched | --precision | --readme
+    python bench.py --cc [--nx 10 --ny 10 --nz 10 --p 3]   (the reference's -cc correctness check)
 """
 from __future__ import annotations
 
@@ -383,8 +385,92 @@ def precision_result(args) -> dict:
 
 
 def run_precision(args) -> int:
-    print(json.dumps(precision_result(args)), flush=True)
+    out = precision_result(args)
+    if args.copy_probe:
+        out["copy_probe"] = copy_probe(1285553088)
+    print(json.dumps(out), flush=True)
     return 0
+
+
+def run_cc(args) -> int:
+    """CorrectnessTest (dawgsMain.cpp:314-416) on an nx x ny x nz box: the
+    reference's own input (mt19937(rank 0), U(-0.25, 0.25), dawgsMain.cpp:
+    326-330), its golden result from gslib's gs_vec (the reference's legacy
+    comparator, compiled from its sources into oracle/_ref, dawgsGS.cpp:34-71),
+    and Error = sum |q - qcheck| (CheckCorrectness, dawgsMain.cpp:29-47) for
+    the device path and for the reference's host loops (the oracle)."""
+    import torch
+    import dawgs_amd as D
+    from dawgs_amd.mesh import make_ids
+    from oracle import oracle as O  # the checker: gslib and the host loops
+    nx, ny, nz, p = args.nx, args.ny, args.nz, args.p
+    ids = make_ids(nx, ny, nz, p)
+    N = len(ids)
+    q = O.reference_q(0, N)
+    if not O.ref_available():
+        log("gslib comparator not built (make -C oracle where the reference tree is present)")
+        return 1
+    qcheck = O.GsRef(ids).gs_vec(q, 1)
+    errors = {}
+
+    def check(name, qtest):
+        bad = np.flatnonzero(np.abs(qtest - qcheck) > 1.0e-6)
+        for n in bad[:10]:
+            log(f"Rank 0, Entry {n}, baseId {ids[n]} q = {qtest[n]:f}, qRef = {qcheck[n]:f}")
+        err = float(np.sum(np.abs(qtest - qcheck)))
+        errors[name] = err
+        log(f"{name}:  Error = {err:g}")
+
+    ogs = D.ogs_t()
+    ogs.Setup(N, ids.copy(), D.Unsigned, D.Auto, False, True)
+    sids = ids.copy()
+    sogs = D.ogs_t()
+    sogs.Setup(N, sids, D.Signed, D.Auto, True, True)
+    hogs, hsogs = O.OracleOgs(ids, O.UNSIGNED), O.OracleOgs(ids, O.SIGNED, True)
+    for tag, dev, host in (("ogs", ogs, hogs), ("sogs", sogs, hsogs)):
+        d = torch.from_numpy(q.copy()).cuda()
+        dev.GatherScatter(d, 1, D.Add, D.Sym)
+        check(f"Device GatherScatter ({tag})", d.cpu().numpy())
+        check(f"Host   GatherScatter ({tag})", host.gather_scatter(q.copy(), 1, O.ADD, O.SYM))
+    d = torch.from_numpy(q.copy()).cuda()
+    gq = torch.zeros(sogs.Ngather, dtype=torch.float64, device="cuda")
+    sogs.Gather(gq, d, 1, D.Add, D.Trans)
+    sogs.Scatter(d, gq, 1, D.NoTrans)
+    check("Device Gather+Scatter", d.cpu().numpy())
+    g = hsogs.gather(q, 1, O.ADD, O.TRANS)
+    check("Host   Gather+Scatter", hsogs.scatter(np.zeros(N), g, 1, O.NOTRANS))
+    out = {"metric": "correctness check against gslib (-cc): Error = sum |q - qcheck|", "errors": errors,
+           "pass": all(e == 0.0 for e in errors.values()), "n_gpus": 1, "dtype": "f64",
+           "data": "the reference's own input: mt19937(0), uniform(-0.25, 0.25)",
+           "config": {"nx": nx, "ny": ny, "nz": nz, "p": p, "local_dofs": N,
+                      "global_dofs": int(ogs.NgatherGlobal)}}
+    print(json.dumps(out), flush=True)
+    return 0 if out["pass"] else 3
+
+
+def copy_probe(nbytes: int) -> dict:
+    """A plain device copy moving `nbytes` (torch.neg: one elementwise kernel,
+    reading nbytes/2 and writing nbytes/2, its name distinct from every other
+    kernel of a run): the practical streaming ceiling, and the known-size
+    kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = nbytes // 16
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    src = torch.empty(n, dtype=torch.float64, device="cuda").uniform_(generator=gen)
+    dst = torch.empty_like(src)
+    st = torch.cuda.current_stream()
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(10):
+        torch.neg(src, out=dst)
+    e1.record(st)
+    torch.cuda.synchronize()
+    cms = e0.elapsed_time(e1) / 10
+    log(f"copy probe: {16 * n} B in {cms * 1e3:.1f} us = {16 * n / (cms * 1e-3) / 1e9:.0f} GB/s")
+    return {"bytes": 16 * n, "us": round(cms * 1e3, 1), "GBps": round(16 * n / (cms * 1e-3) / 1e9, 1)}
 
 
 def idle_probe(call, stream, gaps_ms=(0, 5, 20, 100, 500), n=120) -> dict:
@@ -413,8 +499,15 @@ def main() -> int:
     # the kernel time settles after ~100 calls (~25 ms of work) on a fresh box
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--nx", type=int, default=48)
-    ap.add_argument("--p", type=int, default=7)
+    ap.add_argument("--nx", type=int, default=None, help="elements in x (dawgsSettings.cpp -nx; default 48, "
+                                                         "10 with --cc)")
+    ap.add_argument("--ny", type=int, default=None, help="elements in y (default: nx)")
+    ap.add_argument("--nz", type=int, default=None, help="elements in z (default: nx)")
+    ap.add_argument("--p", type=int, default=None, help="degree (default 7, 3 with --cc)")
+    ap.add_argument("--cc", action="store_true",
+                    help="the reference's correctness check (-cc, dawgsMain.cpp:314-416): the device and host "
+                         "GatherScatter (Unsigned and Signed-unique) and Gather+Scatter against gslib on the "
+                         "reference's own input, its 'Error = ' lines, and one JSON line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only: skip the batched and precision summaries of the default line")
@@ -458,6 +551,12 @@ def main() -> int:
         log("no HIP device visible")
         return 1
     torch.cuda.set_device(0)
+    args.nx = args.nx or (10 if args.cc else 48)
+    args.p = args.p or (3 if args.cc else 7)
+    args.ny = args.ny or args.nx
+    args.nz = args.nz or args.nx
+    if args.cc:
+        return run_cc(args)
     if args.sweep:
         return run_sweep(args)
     if args.batched:
@@ -466,10 +565,11 @@ def main() -> int:
         return run_precision(args)
     if args.readme:
         return run_readme(args)
-    nx, p = args.nx, args.p
+    nx, ny, nz, p = args.nx, args.ny, args.nz, args.p
+    box = f"{nx}^3" if nx == ny == nz else f"{nx}x{ny}x{nz}"
 
     t0 = time.perf_counter()
-    ids = make_ids(nx, nx, nx, p)
+    ids = make_ids(nx, ny, nz, p)
     N = len(ids)
     # Inputs are made before the Setup, so that the warm-ups follow the
     # Setup's own device work without an idle gap: on MI355X a memory-bound
@@ -506,7 +606,7 @@ def main() -> int:
     # measured HBM bytes per launch (two rocprofv3 --pmc runs, scaled by a
     # copy kernel of known size; profiles/k_gs_sym_vec_f64_add_48cubed_p7.md)
     traffic = None
-    if os.path.exists(TRAFFIC_FILE) and (nx, p) == (48, 7):
+    if os.path.exists(TRAFFIC_FILE) and (nx, ny, nz, p) == (48, 48, 48, 7):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -519,24 +619,7 @@ def main() -> int:
         roofline["physical_GBps"] = round(phys, 1)
         roofline["physical_frac"] = round(phys / HBM_PEAK_GBS, 4)
     if args.copy_probe:
-        n = B // 16  # two buffers of B/2 bytes: B bytes moved per copy
-        src = torch.empty(n, dtype=torch.float64, device="cuda").uniform_(generator=gen)
-        dst = torch.empty_like(src)
-        # torch.neg: one elementwise kernel, reads B/2 and writes B/2 bytes, and
-        # its name is distinct from every other kernel of the run
-        for _ in range(3):
-            torch.neg(src, out=dst)
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(10):
-            torch.neg(src, out=dst)
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        cms = ev0.elapsed_time(ev1) / 10
-        roofline["copy_probe"] = {"bytes": 16 * n, "us": round(cms * 1e3, 1),
-                                  "GBps": round(16 * n / (cms * 1e-3) / 1e9, 1)}
-        del src, dst
-        log(f"copy probe: {16 * n} B in {cms * 1e3:.1f} us = {16 * n / (cms * 1e-3) / 1e9:.0f} GB/s")
+        roofline["copy_probe"] = copy_probe(B)
     if args.idle_probe:
         roofline["idle_probe"] = idle_probe(lambda: ogs.GatherScatter(q, 1, D.Add, D.Sym), stream)
     log(f"GPU: {ms * 1e3:.1f} us/call, {achieved:.0f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s), "
@@ -561,10 +644,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"ogs GatherScatter fp64 Add Sym K=1 on a {nx}^3 hex mesh, p={p} "
+        "config": {"workload": f"ogs GatherScatter fp64 Add Sym K=1 on a {box} hex mesh, p={p} "
                                f"({N} local DOFs, {ogs.NgatherGlobal} global), structured numbering, "
                                f"values uniform[-1,1]",
-                   "nx": nx, "p": p, "K": 1, "op": "Add", "trans": "Sym",
+                   "nx": nx, "ny": ny, "nz": nz, "p": p, "K": 1, "op": "Add", "trans": "Sym",
                    "local_dofs": N, "global_dofs": int(ogs.NgatherGlobal), "nnz": ogs.nnzT,
                    "plan_bytes": info["bytes"], "delta_pack_E": info["deltaE"]},
         "roofline": roofline,
