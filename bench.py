@@ -470,7 +470,20 @@ def copy_probe(nbytes: int) -> dict:
     torch.cuda.synchronize()
     cms = e0.elapsed_time(e1) / 10
     log(f"copy probe: {16 * n} B in {cms * 1e3:.1f} us = {16 * n / (cms * 1e-3) / 1e9:.0f} GB/s")
-    return {"bytes": 16 * n, "us": round(cms * 1e3, 1), "GBps": round(16 * n / (cms * 1e-3) / 1e9, 1)}
+    # the same in place (torch.neg_: each element read and written back at
+    # its own address, the access pattern of a GatherScatter's values)
+    for _ in range(3):
+        src.neg_()
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(10):
+        src.neg_()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ims = e0.elapsed_time(e1) / 10
+    log(f"in-place probe: {16 * n} B in {ims * 1e3:.1f} us = {16 * n / (ims * 1e-3) / 1e9:.0f} GB/s")
+    return {"bytes": 16 * n, "us": round(cms * 1e3, 1), "GBps": round(16 * n / (cms * 1e-3) / 1e9, 1),
+            "inplace_us": round(ims * 1e3, 1), "inplace_GBps": round(16 * n / (ims * 1e-3) / 1e9, 1)}
 
 
 def idle_probe(call, stream, gaps_ms=(0, 5, 20, 100, 500), n=120) -> dict:
