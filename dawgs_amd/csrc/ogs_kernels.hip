@@ -137,14 +137,17 @@ __device__ __forceinline__ void comp_add(Comp &a, float x) {
   a.c += e;
 }
 __device__ __forceinline__ float comp_fin(const Comp &a) { return __builtin_isfinite(a.s) ? a.s + a.c : a.s; }
+// comp_add of one member into {0, 0}: s = 0 + x (-0 becomes +0), c = 0.
+// comp_add's c differs only for x inf or NaN, where s stays non-finite from
+// then on and comp_fin ignores c, so the result is the same.
+__device__ __forceinline__ Comp comp_first(float x) { return Comp{0.0f + x, 0.0f}; }
 
 __device__ __forceinline__ float bf16_widen(bf16s x) { return __uint_as_float((uint32_t)x.b << 16); }
 // Round to nearest even; NaN stays NaN (quieted), overflow rounds to inf.
 __device__ __forceinline__ bf16s bf16_round(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16s{(uint16_t)((u >> 16) | 0x40u)};
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return bf16s{(uint16_t)(u >> 16)};
+  const uint32_t u = __float_as_uint(f);
+  const uint32_t r = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  return bf16s{(uint16_t)((u & 0x7fffffffu) > 0x7f800000u ? (u >> 16) | 0x40u : r)};
 }
 constexpr float kBf16Max = 3.38953139e38f;  // 0x7f7f: largest finite bfloat16
 
@@ -229,6 +232,10 @@ template <> struct LRed<bf16s, OGS_MIN> {
   static __device__ __forceinline__ void apply(A &a, float x) { a = (x < a) ? x : a; }
   static __device__ __forceinline__ float fin(A a) { return a; }
 };
+
+// A staged value of the compensated types as float.
+__device__ __forceinline__ float lds_float(float x) { return x; }
+__device__ __forceinline__ float lds_float(f32c x) { return x.v; }
 
 // Whether (T, OP) has kernels: every op for the reference's types; Add,
 // Max and Min for the precision-form storage types.
@@ -795,8 +802,11 @@ struct VecShape {
   static constexpr int VW = sizeof(T) == 8 ? (KT == 3 ? 1 : KT >= 2 ? 2 : 1) : vec_width(KT);
   static constexpr int C = KT / VW;
   static constexpr int WORDS = (int)((sizeof(T) + 3) / 4);
-  static constexpr int SL = VW * WORDS <= 3 ? 8 : VW * WORDS <= 6 ? 4 : 2;
   using L = typename Lds<T>::L;
+  // slots per lane: by item size, and no more than the LDS capacity fills
+  static constexpr int SL0 = VW * WORDS <= 3 ? 8 : VW * WORDS <= 6 ? 4 : 2;
+  static constexpr int SLL = (vec_cap<L>() / KT * C + kBlock - 1) / kBlock;
+  static constexpr int SL = SL0 < SLL ? SL0 : SLL;
   static constexpr int CAP0 = SL * kBlock / C < vec_cap<L>() / KT ? SL * kBlock / C : vec_cap<L>() / KT;
   static constexpr int CAP = CAP0 < kCap ? CAP0 : kCap;
   static_assert(KT % VW == 0, "field count must be a multiple of the item width");
@@ -915,54 +925,135 @@ __device__ __forceinline__ int row_len(const uint32_t *__restrict__ hm, int i) {
   return len + __builtin_ctz(rest);
 }
 
-// Row reduction of a block whose n entries' values (KT fields each) are
-// staged in LDS at their CSR positions with their row-head bytes (the
-// staging barrier already passed): one lane per (row head, field chunk)
-// folds the row's VW fields in CSR order from the op's identity, members read
-// four at a time, and writes the result back over the row (measured faster
-// than every item summing its own row: rows of several members would be read
-// once per member).  Ends with a barrier.
+// Exclusive prefix counts of the head-mask words (one wave): pc[w] = heads
+// in positions [0, 32 w).  An entry's row within its block is then the
+// number of heads at or before its CSR position, minus one (head_row).
+__device__ __forceinline__ void head_prefix(const uint32_t *__restrict__ hm, int n, int32_t *__restrict__ pc) {
+  constexpr int NW = kCap / 32;
+  static_assert(NW == 64, "one wave scans the head-mask words");
+  const int tid = threadIdx.x;
+  if (tid < NW) {
+    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < NW; d <<= 1) {
+      const int y = __shfl_up(incl, d, NW);
+      if (tid >= d) incl += y;
+    }
+    pc[tid] = incl - c;
+  }
+}
+__device__ __forceinline__ int head_row(const uint32_t *__restrict__ hm, const int32_t *__restrict__ pc, int ps) {
+  const int wd = ps >> 5;
+  const uint32_t upto = (uint32_t)((2ull << (ps & 31)) - 1u);  // positions 32 wd .. ps
+  return pc[wd] + __popc(hm[wd] & upto) - 1;
+}
+
+// Fold of one (row, field chunk) of a block staged in LDS: the row's len
+// members from CSR position i, VW fields from chunk c, in CSR order from the
+// op's identity, members read four at a time; the result is written back
+// over the row.
 template <typename T, int OP, int KT>
-__device__ __forceinline__ void vec_reduce(int n, typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
-                                           uint32_t *__restrict__ hm) {
+__device__ __forceinline__ void fold_row(typename Lds<T>::L *__restrict__ val, int i, int c, int len) {
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
-  constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
-  const int ni = n * C;
+  constexpr int VW = VecShape<T, KT>::VW;
+  typename R::A red[VW];
+  const L *src = val + i * KT + c * VW;
+  int mb = 0;
+  if constexpr (std::is_same<typename R::A, Comp>::value) {
+    // compensated sums take the row's first member (len >= 1) in one step
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[v] = comp_first(lds_float(src[v]));
+    mb = 1;
+  } else {
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[v] = R::init();
+  }
+  for (; mb + 4 <= len; mb += 4) {
+    L y[4][VW];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < VW; ++v) y[u][v] = src[(mb + u) * KT + v];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < VW; ++v) R::apply(red[v], y[u][v]);
+  }
+  for (; mb < len; ++mb) {
+#pragma unroll
+    for (int v = 0; v < VW; ++v) R::apply(red[v], src[mb * KT + v]);
+  }
+  L res[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) res[v] = R::fin(red[v]);
+  L *dst = val + i * KT + c * VW;
+  for (int r = 0; r < len; ++r)
+#pragma unroll
+    for (int v = 0; v < VW; ++v) dst[r * KT + v] = res[v];
+}
+
+// LDS of a row-compacted reduction: the head-mask prefix counts and the row
+// count (pc), and (rp) the head position of every row plus the end, n, 16 bits
+// each.  rp overlays the block's head bytes, which are dead once the head
+// mask is built; the head region is sized for both (head_bytes).
+struct RowIndex {
+  int32_t *pc;  // kCap / 32 + 1 words, the last the row count
+  uint16_t *rp;
+};
+// Whether a shape's blocks reduce over compacted rows: those below kCap
+// entries, whose row list fits the head region with at most 700 more bytes
+// (so the kernels keep their resident workgroups per CU); measured on
+// bfloat16 K = 3, instruction bound, whose vector instructions fell by a
+// quarter.
+template <typename T, int KT> __host__ __device__ constexpr bool rows_reduce() { return VecShape<T, KT>::CAP < kCap; }
+template <typename T, int KT> __host__ __device__ constexpr int head_bytes() {
+  return rows_reduce<T, KT>() && 2 * (VecShape<T, KT>::CAP + 2) > kCap ? 2 * (VecShape<T, KT>::CAP + 2) : kCap;
+}
+
+// Row reduction of a block whose n entries' values (KT fields each) are
+// staged in LDS at their CSR positions with their row-head bytes (the
+// staging barrier already passed), each (row, field chunk) folded by one lane
+// (measured faster than every item summing its own row: rows of several
+// members would be read once per member).  Without a row index, lanes step
+// over the entries and fold at the heads; with one (ri), the heads' positions
+// are first compacted in CSR order, so lanes step over rows and none idles on
+// an entry that is not a head.  Ends with a barrier.
+template <typename T, int OP, int KT>
+__device__ __forceinline__ void vec_reduce(int n, typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                           uint32_t *__restrict__ hm, const RowIndex *ri = nullptr) {
+  constexpr int C = VecShape<T, KT>::C;
   const int tid = threadIdx.x;
   head_mask(head, n, hm);
   __syncthreads();
-  for (int t = tid; t < ni; t += kBlock) {
-    const int i = t / C, c = t - i * C;
-    if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
-    const int len = row_len(hm, i);
-    typename R::A red[VW];
-#pragma unroll
-    for (int v = 0; v < VW; ++v) red[v] = R::init();
-    const L *src = val + i * KT + c * VW;
-    int mb = 0;
-    for (; mb + 4 <= len; mb += 4) {
-      L y[4][VW];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) y[u][v] = src[(mb + u) * KT + v];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) R::apply(red[v], y[u][v]);
+  if (!ri) {
+    for (int t = tid; t < n * C; t += kBlock) {
+      const int i = t / C, c = t - i * C;
+      if (!((hm[i >> 5] >> (i & 31)) & 1u)) continue;
+      fold_row<T, OP, KT>(val, i, c, row_len(hm, i));
     }
-    for (; mb < len; ++mb) {
-#pragma unroll
-      for (int v = 0; v < VW; ++v) R::apply(red[v], src[mb * KT + v]);
+    __syncthreads();
+    return;
+  }
+  head_prefix(hm, n, ri->pc);
+  __syncthreads();
+  for (int p = tid; p < n; p += kBlock) {
+    const int wd = p >> 5;
+    const uint32_t bits = hm[wd];
+    const int incl = ri->pc[wd] + __popc(bits & (uint32_t)((2ull << (p & 31)) - 1u));  // heads in [0, p]
+    if ((bits >> (p & 31)) & 1u) ri->rp[incl - 1] = (uint16_t)p;
+    if (p == n - 1) {  // position 0 is a head, so incl here is the block's row count
+      ri->rp[incl] = (uint16_t)n;
+      ri->pc[kCap / 32] = incl;
     }
-    L res[VW];
-#pragma unroll
-    for (int v = 0; v < VW; ++v) res[v] = R::fin(red[v]);
-    L *dst = val + i * KT + c * VW;
-    for (int r = 0; r < len; ++r)
-#pragma unroll
-      for (int v = 0; v < VW; ++v) dst[r * KT + v] = res[v];
+  }
+  __syncthreads();
+  const int nr = n > 0 ? ri->pc[kCap / 32] : 0;
+  for (int u = tid; u < nr * C; u += kBlock) {
+    const int r = u / C, c = u - r * C;
+    const int i = ri->rp[r];
+    fold_row<T, OP, KT>(val, i, c, (int)ri->rp[r + 1] - i);
   }
   __syncthreads();
 }
@@ -975,7 +1066,8 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
                                            const int32_t *__restrict__ starts,
                                            const T (&x)[SL][VecShape<T, KT>::VW],
                                            typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
-                                           uint32_t *__restrict__ hm, uint32_t nbits = 0u) {
+                                           uint32_t *__restrict__ hm, uint32_t nbits = 0u,
+                                           const RowIndex *ri = nullptr) {
   constexpr int VW = VecShape<T, KT>::VW, C = VecShape<T, KT>::C;
   const int n = s.n;
   if (n == 0) return;
@@ -1018,7 +1110,7 @@ __device__ __forceinline__ void vec_finish(const VecStage<SL> &s, const VecMeta 
     }
   }
   __syncthreads();
-  vec_reduce<T, OP, KT>(n, val, head, hm);
+  vec_reduce<T, OP, KT>(n, val, head, hm, ri);
   T *q = static_cast<T *>(s.src.q);
 #pragma unroll
   for (int j = 0; j < SL; ++j) {
@@ -1043,7 +1135,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   using V = VecShape<T, KT>;
   using L = typename Lds<T>::L;
   __shared__ L val[vec_cap<L>()];
-  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t head4[(head_bytes<T, KT>() + 3) / 4];
   __shared__ uint32_t hm[kCap / 32 + 1];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const VecSrc src = {part, ids, pk, base, q, gUseN, stride};
@@ -1055,7 +1147,13 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_vec(const int4 *__restrict__ 
   vec_meta(src, b, m);
   vec_index<T, KT>(s, src, b, m);
   vec_gather<T, KT>(s, x);
-  vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
+  if constexpr (rows_reduce<T, KT>()) {
+    __shared__ int32_t pc[kCap / 32 + 1];
+    const RowIndex ri = {pc, reinterpret_cast<uint16_t *>(head4)};
+    vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm, 0u, &ri);
+  } else {
+    vec_finish<T, OP, KT>(s, m, starts, x, val, head, hm);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1245,30 +1343,6 @@ __global__ __launch_bounds__(kBlock) void k_gs_trans_mask(const int4 *__restrict
 #pragma unroll
   for (int j = 0; j < V::SL; ++j) nbits |= ((mw[j] >> (slot_col(s.w[j], s.bb, packed) & 31)) & 1u) << j;
   vec_finish<T, OP, KT, V::SL, true>(s, m, starts, x, val, head, hm, nbits);
-}
-
-// Exclusive prefix counts of the head-mask words (one wave): pc[w] = heads
-// in positions [0, 32 w).  An entry's row within its block is then the
-// number of heads at or before its CSR position, minus one (head_row).
-__device__ __forceinline__ void head_prefix(const uint32_t *__restrict__ hm, int n, int32_t *__restrict__ pc) {
-  constexpr int NW = kCap / 32;
-  static_assert(NW == 64, "one wave scans the head-mask words");
-  const int tid = threadIdx.x;
-  if (tid < NW) {
-    const int c = tid < ((n + 31) >> 5) ? __popc(hm[tid]) : 0;
-    int incl = c;
-#pragma unroll
-    for (int d = 1; d < NW; d <<= 1) {
-      const int y = __shfl_up(incl, d, NW);
-      if (tid >= d) incl += y;
-    }
-    pc[tid] = incl - c;
-  }
-}
-__device__ __forceinline__ int head_row(const uint32_t *__restrict__ hm, const int32_t *__restrict__ pc, int ps) {
-  const int wd = ps >> 5;
-  const uint32_t upto = (uint32_t)((2ull << (ps & 31)) - 1u);  // positions 32 wd .. ps
-  return pc[wd] + __popc(hm[wd] & upto) - 1;
 }
 
 // Row heads of a block as bytes at CSR positions: from the pack words, or
@@ -1778,7 +1852,7 @@ __global__ __launch_bounds__(kBlock) void k_gs2_vec(const int4 *__restrict__ par
 template <typename T, int OP, int KT>
 __device__ __forceinline__ void batch_item(const BatchDev *__restrict__ descs, int2 w,
                                            typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
-                                           uint32_t *__restrict__ hm) {
+                                           uint32_t *__restrict__ hm, int32_t *__restrict__ pc) {
   using V = VecShape<T, KT>;
   const BatchDev d = load_desc(descs, w.x);
   const VecSrc src = {d.part, d.G.ids, d.G.pk, d.G.base, d.q, 0, d.G.stride};
@@ -1788,7 +1862,21 @@ __device__ __forceinline__ void batch_item(const BatchDev *__restrict__ descs, i
   vec_meta(src, w.y, m);
   vec_index<T, KT>(s, src, w.y, m);
   vec_gather<T, KT>(s, x);
-  vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head, hm);
+  if constexpr (rows_reduce<T, KT>()) {
+    const RowIndex ri = {pc, reinterpret_cast<uint16_t *>(head)};
+    vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head, hm, 0u, &ri);
+  } else {
+    vec_finish<T, OP, KT>(s, m, d.G.starts, x, val, head, hm);
+  }
+}
+
+// Head-region bytes of a kernel running every batched field-count class.
+template <typename T> __host__ __device__ constexpr int batch_head_bytes() {
+  constexpr int b[6] = {head_bytes<T, 1>(), head_bytes<T, 2>(), head_bytes<T, 3>(),
+                        head_bytes<T, 4>(), head_bytes<T, 8>(), head_bytes<T, 16>()};
+  int m = 0;
+  for (int i = 0; i < 6; ++i) m = b[i] > m ? b[i] : m;
+  return m;
 }
 
 // One node per field-count class: the items of each XCD form one contiguous
@@ -1798,9 +1886,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_vec(const BatchDev *__restrict
                                                       const int2 *__restrict__ items) {
   using L = typename Lds<T>::L;
   __shared__ L val[vec_cap<L>()];
-  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t head4[(head_bytes<T, KT>() + 3) / 4];
   __shared__ uint32_t hm[kCap / 32 + 1];
-  batch_item<T, OP, KT>(descs, items[xcd_block(blockIdx.x, gridDim.x)], val, reinterpret_cast<uint8_t *>(head4), hm);
+  __shared__ int32_t pc[kCap / 32 + 1];
+  batch_item<T, OP, KT>(descs, items[xcd_block(blockIdx.x, gridDim.x)], val, reinterpret_cast<uint8_t *>(head4), hm,
+                        pc);
 }
 
 // All classes in one node: workgroup b runs item b of the concatenated class
@@ -1814,17 +1904,18 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(const BatchDev *__restri
                                                         const int2 *__restrict__ items, ClassEnds ce) {
   using L = typename Lds<T>::L;
   __shared__ L val[vec_cap<L>()];
-  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t head4[(batch_head_bytes<T>() + 3) / 4];
   __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32 + 1];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int2 w = items[b];
-  if (b < ce.end[0]) batch_item<T, OP, 1>(descs, w, val, head, hm);
-  else if (b < ce.end[1]) batch_item<T, OP, 2>(descs, w, val, head, hm);
-  else if (b < ce.end[2]) batch_item<T, OP, 3>(descs, w, val, head, hm);
-  else if (b < ce.end[3]) batch_item<T, OP, 4>(descs, w, val, head, hm);
-  else if (b < ce.end[4]) batch_item<T, OP, 8>(descs, w, val, head, hm);
-  else batch_item<T, OP, 16>(descs, w, val, head, hm);
+  if (b < ce.end[0]) batch_item<T, OP, 1>(descs, w, val, head, hm, pc);
+  else if (b < ce.end[1]) batch_item<T, OP, 2>(descs, w, val, head, hm, pc);
+  else if (b < ce.end[2]) batch_item<T, OP, 3>(descs, w, val, head, hm, pc);
+  else if (b < ce.end[3]) batch_item<T, OP, 4>(descs, w, val, head, hm, pc);
+  else if (b < ce.end[4]) batch_item<T, OP, 8>(descs, w, val, head, hm, pc);
+  else batch_item<T, OP, 16>(descs, w, val, head, hm, pc);
 }
 
 // ---------------------------------------------------------------------------
