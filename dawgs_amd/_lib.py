@@ -20,7 +20,8 @@ class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
                  "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN", "deltaE", "deltaFallback",
-                 "ownerMap", "nMask")] + \
+                 "ownerMap", "nMask", "bundleRy", "bundleRz", "bundleTy", "bundleTz", "bundleTileD",
+                 "bundleFallback")] + \
                [("bytes", C.c_int64)]
 
 

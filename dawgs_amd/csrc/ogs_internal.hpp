@@ -68,15 +68,36 @@ struct Pack {
   int32_t plain = 0;   // blocks left in CSR order
 };
 
-// Delta pack of the T list over partition 0 (K = 1 one-list GatherScatter,
+// Delta pack of the T list over a partition (K = 1 one-list GatherScatter,
 // k_gs_sym_delta): per block kCap 16-bit words (CSR position | row-head bit
-// | delta code), and kCap / 512 * E 32-bit exception words.
+// | delta code), and kCap / 512 * E 32-bit exception words.  A fallback
+// block (one whose columns the 16-bit words cannot encode) carries the
+// fallback mark as every range's exception word 0 and its slot in the
+// 32-bit words `f` as word 1: slot b in the partition's full pack, or, once
+// compacted, its rank among the fallback blocks in a pack holding only
+// theirs (ownsF).
 struct DeltaPack {
   uint16_t *w = nullptr;               // nblocks * kCap words
   uint32_t *x = nullptr;               // nblocks * (kCap / 512) * E exception words
   int32_t E = 0;                       // exception words per 512-entry range
   uint32_t tbl[4] = {0u, 0u, 0u, 0u};  // byte c: the delta of code c
-  int32_t fallback = 0;                // blocks that run from the 32-bit pack
+  int32_t fallback = 0;                // blocks that run from the 32-bit words
+  Pack f;                              // the fallback blocks' 32-bit words
+  bool ownsF = false;                  // f is the compacted copy (else the partition's pack)
+};
+
+// The K = 1 one-list GatherScatter over bundled rows: the T list's rows
+// reordered so that a block holds the rows of ty neighbouring lines and tz
+// neighbouring layers of a structured mesh side by side (row r's bundle
+// position from the line and layer row strides Ry, Rz that the plan detects
+// in first-encounter order), cut on entries like partition 0, with its own
+// delta pack and block order.  GatherScatter results do not depend on which
+// block a row falls in (each row is reduced in its own CSR order), so only
+// that launch uses it; the plan keeps it where it measured faster.
+struct Bundle {
+  Partition part;   // entries index the bundled CSR, which is not kept
+  DeltaPack dp;     // compacted fallback words (no block left in CSR order)
+  int32_t Ry = 0, Rz = 0, ty = 0, tz = 0;
 };
 
 }  // namespace ogs_hip
@@ -110,6 +131,7 @@ struct ogs_hip_plan_s {
   ogs_hip::Pack packNN;
   ogs_hip::Pack packS;  // the T list over partS[0] (a copy of packT[0] when partS is partT)
   ogs_hip::DeltaPack dpack;  // the T list over partT[0] with 16-bit words (w null: none)
+  ogs_hip::Bundle bundle;    // bundled K = 1 GatherScatter blocks (part.d null: none)
   // T lists cut on entries into blocks of at most kCap3[i] entries, with
   // their packs: the one-list GatherScatter with three fields runs whole
   // blocks of its shape instead of the next power-of-two width below it
@@ -151,6 +173,11 @@ struct DeviceSetup {
 // into h_ids.  Returns OGS_E_HALO_POSITIVE for a Halo group without exactly
 // one positive id, or a hipError_t.
 int device_setup(int32_t N, int64_t *h_ids, int kind, int unique, hipStream_t st, DeviceSetup *out);
+// The T list (rs, ci) with its rows reordered into bundles of ty lines and
+// tz layers (row strides Ry, Rz): rsP/ciP allocated here, row p of the
+// result being the list's row at the p-th smallest bundle key.
+int bundle_csr(const int32_t *rs, const int32_t *ci, int32_t nrows, int32_t nnz, int32_t Ry, int32_t Rz, int ty,
+               int tz, hipStream_t st, int32_t **rsP, int32_t **ciP);
 // Makes `plan` hold `keep` (device arrays it reads) until it is freed.
 inline void plan_keep(ogs_hip_plan_s *plan, std::shared_ptr<void> keep) { plan->keep = std::move(keep); }
 }

@@ -51,6 +51,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstddef>
+#include <cstdlib>
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -845,6 +846,14 @@ __device__ __forceinline__ void vec_meta(const VecSrc &src, int b, VecMeta &m) {
   m.bb = src.pk ? src.base[b] : make_int4(-1, 0, 0, 0);
 }
 
+// The same for a block whose pack words sit at another slot than its block
+// id (a delta pack's fallback block in a pack of fallback blocks only).
+__device__ __forceinline__ void vec_meta_at(const VecSrc &src, int b, int slot, VecMeta &m) {
+  m.p0 = src.part[b];
+  m.p1 = src.part[b + 1];
+  m.bb = src.pk ? src.base[slot] : make_int4(-1, 0, 0, 0);
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Index words of block b.  With a pack the words sit at b * stride
@@ -1268,13 +1277,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict_
   const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
   const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + wv) * E + lane) : 0u;
   if ((uint32_t)uniform((int)ex) == kDFallback) {
-    // this block runs from the 32-bit pack (every range carries the mark)
+    // this block runs from its 32-bit words (every range carries the mark,
+    // and the words' slot as exception word 1)
+    const int slot = __builtin_amdgcn_readlane((int)ex, 1);
     const VecSrc src = {part, ids, pk, base, q, 0, stride};
     VecStage<V::SL> s;
     VecMeta m;
     T x[V::SL][1];
-    vec_meta(src, b, m);
-    vec_index<T, 1>(s, src, b, m);
+    vec_meta_at(src, b, slot, m);
+    vec_index<T, 1>(s, src, slot, m);
     vec_gather<T, 1>(s, x);
     vec_finish<T, OP, 1>(s, m, starts, x, val, head, hm);
     return;
@@ -1412,18 +1423,20 @@ __device__ __forceinline__ void gather_rows(int n, int r0, const typename Lds<T>
   }
 }
 
+// Block b of a Gather whose pack words sit at `slot` (b, or a delta pack's
+// fallback block's slot among the fallback blocks' words).
 template <typename T, int OP, int KT>
-__device__ __forceinline__ void gather_vec_block(const VecSrc &src, int b, const int32_t *__restrict__ starts,
-                                                 T *__restrict__ gq, typename Lds<T>::L *__restrict__ val,
-                                                 uint8_t *__restrict__ head, uint32_t *__restrict__ hm,
-                                                 int32_t *__restrict__ pc) {
+__device__ __forceinline__ void gather_vec_block(const VecSrc &src, int b, int slot,
+                                                 const int32_t *__restrict__ starts, T *__restrict__ gq,
+                                                 typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                                 uint32_t *__restrict__ hm, int32_t *__restrict__ pc) {
   using V = VecShape<T, KT>;
   constexpr int SL = V::SL, VW = V::VW, C = V::C;
   VecStage<SL> s;
   VecMeta m;
   T x[SL][VW];
-  vec_meta(src, b, m);
-  vec_index<T, KT>(s, src, b, m);
+  vec_meta_at(src, b, slot, m);
+  vec_index<T, KT>(s, src, slot, m);
   vec_gather<T, KT>(s, x);
   const int n = s.n;
   if (n == 0) return;
@@ -1468,7 +1481,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
   const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   const int b = order ? order[xb] : xb;
-  gather_vec_block<T, OP, KT>(src, b, starts, gq, val, reinterpret_cast<uint8_t *>(head4), hm, pc);
+  gather_vec_block<T, OP, KT>(src, b, b, starts, gq, val, reinterpret_cast<uint8_t *>(head4), hm, pc);
 }
 
 // Gather K = 1 over partition 0 from the delta pack (see k_gs_sym_delta):
@@ -1500,8 +1513,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_delta(const int4 *__restrict_
   const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
   const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + wv) * E + lane) : 0u;
   if ((uint32_t)uniform((int)ex) == kDFallback) {
+    const int slot = __builtin_amdgcn_readlane((int)ex, 1);
     const VecSrc src = {part, ids, pk, base, const_cast<T *>(q), 0, stride};
-    gather_vec_block<T, OP, 1>(src, b, starts, gq, val, head, hm, pc);
+    gather_vec_block<T, OP, 1>(src, b, slot, starts, gq, val, head, hm, pc);
     return;
   }
   const int r0 = uniform(p0.x);
@@ -2386,6 +2400,7 @@ __global__ __launch_bounds__(kBlock) void k_build_delta(const int4 *__restrict__
   uint32_t *xw = dx + ((size_t)b * kDRanges + w) * E;
   if (bb.x < 0 || over) {
     if (lane == 0) xw[0] = kDFallback;
+    if (lane == 1) xw[1] = (uint32_t)b;  // its words' slot: b in the partition's pack
     if (tid == 0) atomicAdd(fallbacks, 1);
     return;
   }
@@ -2413,6 +2428,25 @@ __global__ __launch_bounds__(kBlock) void k_build_delta(const int4 *__restrict__
     run += __builtin_popcountll(mk);
     dw[(size_t)b * kCap + w * kDWave + lane * (kDWave / 64) + j] = (uint16_t)word;
   }
+}
+
+// The fallback blocks' 32-bit words in a pack of their own: each fallback
+// block of the delta pack takes the next slot (written as every range's
+// exception word 1), its words and segment bases copied there.
+__global__ __launch_bounds__(kBlock) void k_compact_fallback(uint32_t *__restrict__ dx, int E,
+                                                             const uint32_t *__restrict__ pk,
+                                                             const int4 *__restrict__ base, int stride,
+                                                             uint32_t *__restrict__ fpk, int4 *__restrict__ fbase,
+                                                             int32_t *__restrict__ count) {
+  __shared__ int slot;
+  const int b = blockIdx.x;
+  if (dx[(size_t)b * kDRanges * E] != kDFallback) return;
+  if (threadIdx.x == 0) slot = atomicAdd(count, 1);
+  __syncthreads();
+  const int sl = slot;
+  if (threadIdx.x < kDRanges) dx[((size_t)b * kDRanges + threadIdx.x) * E + 1] = (uint32_t)sl;
+  if (threadIdx.x == 0) fbase[sl] = base[b];
+  for (int i = threadIdx.x; i < stride; i += kBlock) fpk[(size_t)sl * stride + i] = pk[(size_t)b * stride + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -2689,16 +2723,28 @@ static int launch_gs_owner(int op, int K, const Plan *p, T *v, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
-// The K = 1 one-list GatherScatter over partition 0 from the delta pack.
+// The K = 1 one-list GatherScatter from a delta pack over partition pt
+// (CSR (rs, ci) of the list pt indexes: read only by blocks left in CSR
+// order, which a bundle has none of).
+template <typename T, int OP>
+static void launch_delta_on(const Partition &pt, const DeltaPack &d, const int32_t *rs, const int32_t *ci,
+                            int ncols, T *v, const int *order, hipStream_t st) {
+  hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk, d.f.base,
+                     d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
+                     make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), ncols, v, order);
+}
+
+// ... over the plan's bundled blocks where it has them, else partition 0.
 template <typename T, int OP>
 static void launch_delta_op(const Plan *p, T *v, hipStream_t st) {
+  const Bundle &u = p->bundle;
+  if (u.part.d) {
+    launch_delta_on<T, OP>(u.part, u.dp, nullptr, nullptr, p->Ncols, v, (u.part.tiled & kTiledGS) ? u.part.order : nullptr,
+                           st);
+    return;
+  }
   const Partition &pt = p->partT[0];
-  const Pack &k = p->packT[0];
-  const DeltaPack &d = p->dpack;
-  hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT, k.pk,
-                     k.base, k.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
-                     make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), p->Ncols, v,
-                     (pt.tiled & kTiledGS) ? pt.order : nullptr);
+  launch_delta_on<T, OP>(pt, p->dpack, p->rsT, p->ciT, p->Ncols, v, (pt.tiled & kTiledGS) ? pt.order : nullptr, st);
 }
 
 template <typename T>
@@ -2871,10 +2917,9 @@ static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, cons
 template <typename T, int OP>
 static void launch_gather_delta_op(const Plan *p, const T *v, T *gv, hipStream_t st) {
   const Partition &pt = p->partT[0];
-  const Pack &k = p->packT[0];
   const DeltaPack &d = p->dpack;
-  hipLaunchKernelGGL((k_gather_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT, k.pk,
-                     k.base, k.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
+  hipLaunchKernelGGL((k_gather_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT, d.f.pk,
+                     d.f.base, d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
                      make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), p->Ncols, v, gv,
                      (pt.tiled & kTiledGather) ? pt.order : nullptr);
 }
@@ -3091,11 +3136,12 @@ static int time_launches(F launch, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
   return rc;
 }
 
-// Whether `order` makes `launch(order)` at least 5% faster than block id
-// order: best of three passes each, the candidates' order alternating (the
-// first launches after a setup run slow).
+// Whether `order` makes `launch(order)` faster than block id order by the
+// factor `by` (5% unless given): best of three passes each, the candidates'
+// order alternating (the first launches after a setup run slow).
 template <class F>
-static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool *pays) {
+static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool *pays,
+                      float by = 0.95f) {
   float best[2] = {0.f, 0.f};
   int rc = 0;
   for (int pass = 0; pass < 3 && !rc; ++pass) {
@@ -3106,14 +3152,14 @@ static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0,
       if (!rc && (best[c] == 0.f || us < best[c])) best[c] = us;
     }
   }
-  *pays = !rc && best[1] < 0.95f * best[0];
+  *pays = !rc && best[1] < by * best[0];
   return rc;
 }
 
 // The most common block distance D >= kTileDMin between the blocks of
-// adjacent columns of the T list over partition `pt`, or 0 when none covers
-// 0.5% of the columns.
-static int layer_distance(const Plan *p, const Partition &pt, hipStream_t st, int *D) {
+// adjacent columns of the T list over partition `pt` (column ids `ids` in
+// the order pt indexes), or 0 when none covers 0.5% of the columns.
+static int layer_distance(const Plan *p, const Partition &pt, const int32_t *ids, hipStream_t st, int *D) {
   *D = 0;
   int32_t *blk = nullptr;
   uint32_t *hist = nullptr;
@@ -3127,7 +3173,7 @@ static int layer_distance(const Plan *p, const Partition &pt, hipStream_t st, in
   if (!rc) rc = (int)hipMemsetAsync(blk, 0xff, sizeof(int32_t) * (size_t)p->Ncols, st);
   if (!rc) rc = (int)hipMemsetAsync(hist, 0, sizeof(uint32_t) * kTileDMax, st);
   if (!rc) {
-    hipLaunchKernelGGL(k_col_block, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->ciT, blk);
+    hipLaunchKernelGGL(k_col_block, dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, ids, blk);
     hipLaunchKernelGGL(k_block_gaps, dim3(grid_for(p->Ncols, 1024)), dim3(kBlock), 0, st, blk, p->Ncols, hist);
     rc = (int)hipGetLastError();
   }
@@ -3158,8 +3204,8 @@ static int tile_schedule(Plan *p, hipStream_t st) {
   // scatter by columns)
   const bool rowScatter = ps.d != pt.d && ps.maxT <= kCap;
   int DT = 0, DS = 0;
-  int rc = layer_distance(p, pt, st, &DT);
-  if (!rc && rowScatter) rc = layer_distance(p, ps, st, &DS);
+  int rc = layer_distance(p, pt, p->ciT, st, &DT);
+  if (!rc && rowScatter) rc = layer_distance(p, ps, p->ciT, st, &DS);
   if (rc || (!DT && !DS)) return rc;
   double *vals = nullptr, *rows = nullptr;
   int *oT = nullptr, *oS = nullptr;
@@ -3193,7 +3239,7 @@ static int tile_schedule(Plan *p, hipStream_t st) {
       rc = order_pays([&](const int *o) {
         if (dp.w)
           hipLaunchKernelGGL((k_gs_sym_delta<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
-                             p->ciT, k.pk, k.base, k.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, o);
+                             p->ciT, dp.f.pk, dp.f.base, dp.f.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, o);
         else
           hipLaunchKernelGGL((k_gs_sym_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d,
                              p->rsT, p->ciT, k.pk, k.base, k.stride, 0, vals, o);
@@ -3202,7 +3248,7 @@ static int tile_schedule(Plan *p, hipStream_t st) {
       rc = order_pays([&](const int *o) {
         if (dp.w)
           hipLaunchKernelGGL((k_gather_delta<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
-                             p->ciT, k.pk, k.base, k.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, rows, o);
+                             p->ciT, dp.f.pk, dp.f.base, dp.f.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, rows, o);
         else
           hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d,
                              p->rsT, p->ciT, k.pk, k.base, k.stride, vals, rows, o);
@@ -3281,8 +3327,8 @@ static int build_n_mask(Plan *p, int32_t nrows, int32_t *scratch, hipStream_t st
   return 0;
 }
 
-static int build_partition(Plan *p, bool cutOnN, bool rowsInKey, int nrows, int W, int32_t *scratch, hipStream_t st,
-                           Partition *out) {
+static int build_partition(const Plan *p, const int32_t *rsT, const int32_t *rsN, bool cutOnN, bool rowsInKey,
+                           int nrows, int W, int32_t *scratch, hipStream_t st, Partition *out) {
   const int64_t keys = (int64_t)(cutOnN ? p->nnzN : p->nnzT) + (rowsInKey ? nrows : 0);
   const int64_t nb = (keys + W - 1) / W;
   if (nb > INT32_MAX - 1) return OGS_E_ARG;
@@ -3291,7 +3337,7 @@ static int build_partition(Plan *p, bool cutOnN, bool rowsInKey, int nrows, int 
   out->nblocks = (int)nb;
   hipError_t e = hipMalloc(&out->d, sizeof(int4) * (size_t)(nb + 1));
   if (e != hipSuccess) return (int)e;
-  hipLaunchKernelGGL(k_partition, dim3(grid_for(nb + 1, 1 << 30)), dim3(kBlock), 0, st, p->rsT, p->rsN,
+  hipLaunchKernelGGL(k_partition, dim3(grid_for(nb + 1, 1 << 30)), dim3(kBlock), 0, st, rsT, rsN,
                      cutOnN ? 1 : 0, rowsInKey ? 1 : 0, nrows, W, (int)nb, out->d);
   e = hipMemsetAsync(scratch, 0, 3 * sizeof(int32_t), st);
   if (e != hipSuccess) return (int)e;
@@ -3332,10 +3378,55 @@ static int build_pack(const Partition &pt, int nrows, const int32_t *starts, con
   return OGS_SUCCESS;
 }
 
+static void free_pack(Pack &k) {
+  if (k.pk) (void)hipFree(k.pk);
+  if (k.base) (void)hipFree(k.base);
+  k = Pack();
+}
+
 static void free_delta(DeltaPack &d) {
   if (d.w) (void)hipFree(d.w);
   if (d.x) (void)hipFree(d.x);
+  if (d.ownsF) free_pack(d.f);
   d = DeltaPack();
+}
+
+// Moves the fallback blocks' 32-bit words of delta pack d (over a partition
+// of nblocks blocks, borrowing that partition's pack) into a pack of their
+// own, so that the partition's pack can go.  False (d unchanged) when the
+// device has no room.
+static int compact_fallback(DeltaPack &d, int nblocks, int32_t *scratch, hipStream_t st, bool *done) {
+  *done = false;
+  if (!d.w || d.ownsF) return 0;
+  const Pack k = d.f;
+  Pack c;
+  c.stride = k.stride;
+  c.packed = d.fallback;
+  if (d.fallback > 0 &&
+      (hipMalloc(&c.pk, sizeof(uint32_t) * (size_t)d.fallback * (size_t)k.stride) != hipSuccess ||
+       hipMalloc(&c.base, sizeof(int4) * (size_t)d.fallback) != hipSuccess)) {
+    (void)hipGetLastError();
+    free_pack(c);
+    return 0;
+  }
+  int rc = 0;
+  if (d.fallback > 0) {
+    rc = (int)hipMemsetAsync(scratch, 0, sizeof(int32_t), st);
+    if (!rc) {
+      hipLaunchKernelGGL(k_compact_fallback, dim3(nblocks), dim3(kBlock), 0, st, d.x, d.E, k.pk, k.base, k.stride,
+                         c.pk, c.base, scratch);
+      rc = (int)hipGetLastError();
+    }
+    if (!rc) rc = (int)hipStreamSynchronize(st);
+  }
+  if (rc) {
+    free_pack(c);
+    return rc;
+  }
+  d.f = c;
+  d.ownsF = true;
+  *done = true;
+  return 0;
 }
 
 // The delta pack of the T list over partition 0, from its 32-bit pack:
@@ -3345,10 +3436,8 @@ static void free_delta(DeltaPack &d) {
 // its delta words), then the
 // words.  Built for entry-cut partitions that fill the device at K = 1 (so
 // that the K = 1 launches use partition 0); none when the device has no room.
-static int build_delta_pack(Plan *p, int32_t *scratch, hipStream_t st) {
-  const Partition &pt = p->partT[0];
-  const Pack &k = p->packT[0];
-  DeltaPack &dp = p->dpack;
+static int build_delta_pack(const Plan *p, const Partition &pt, const Pack &k, DeltaPack &dp, int32_t *scratch,
+                            hipStream_t st) {
   if (!k.pk || k.packed == 0 || pt.rowsInKey || pt.maxT > kCap || pt.nblocks < kFillPerCu * p->cus) return 0;
   uint32_t *hist = nullptr;
   if (hipMalloc(&hist, sizeof(uint32_t) * 256) != hipSuccess) { (void)hipGetLastError(); return 0; }
@@ -3406,15 +3495,308 @@ static int build_delta_pack(Plan *p, int32_t *scratch, hipStream_t st) {
   if (!rc) rc = (int)hipMemcpyAsync(&fb, scratch, sizeof(fb), hipMemcpyDeviceToHost, st);
   if (!rc) rc = (int)hipStreamSynchronize(st);
   dp.fallback = fb;
+  dp.f = k;  // fallback blocks read the partition's pack (slot = block id)
+  dp.ownsF = false;
   // mostly fallback blocks (columns without short runs): the 32-bit pack serves
   if (!rc && 4 * (int64_t)fb > pt.nblocks) free_delta(dp);
   return rc;
 }
 
-static void free_pack(Pack &k) {
-  if (k.pk) (void)hipFree(k.pk);
-  if (k.base) (void)hipFree(k.base);
-  k = Pack();
+// ---------------------------------------------------------------------------
+// Bundled GatherScatter blocks (Plan::bundle).
+//
+// On a structured hex mesh the T list's rows, in first-encounter order, run
+// along lines of the mesh: row r + Ry is r's neighbour in the next line and
+// r + Rz in the next layer.  At low degree most rows are shared vertex and
+// edge nodes whose members lie in four (or eight) elements of neighbouring
+// lines and layers, so blocks of consecutive rows (one line segment each)
+// read and write each element's values in quarters, from four blocks: four
+// times the cache lines per wave instruction, and lines written back in
+// parts.  A bundle puts ty lines and tz layers of rows side by side (row r
+// at (bundle, position along the line, layer, line)), so a block consumes
+// whole elements.  48^3-element-equivalent meshes (~56 M entries, fp64):
+// 192^3 p=1 281 -> 241 us and 128^3 p=2 306 -> ~270 us per GatherScatter,
+// measured with strides from the mesh; no gain at p >= 3.
+
+constexpr int kBundleSample = 4096;      // rows whose shifts are compared
+constexpr int kBundleWindow = 1 << 20;   // columns whose row distances are counted
+constexpr int kBundleMinLen = 3;         // bundles for lists of >= 3 entries per row
+
+namespace {
+// Host copy of rows [r0, r0 + n) of a CSR list.
+struct RowSample {
+  int32_t r0 = 0, n = 0;
+  std::vector<int32_t> rs, ci;  // rs[i] - rs[0] indexes ci
+  int first(int i) const { return ci[rs[i] - rs[0]]; }
+};
+}  // namespace
+
+// Fraction of sample rows r in [0, S) for which row r + L's first column
+// minus row r's is the commonest such difference (a lattice shift of a
+// structured numbering gives one difference for all but its edge rows).
+static double shift_freq(const RowSample &rw, int S, int L) {
+  if (L <= 0 || S + L > rw.n) return 0.0;
+  std::vector<int64_t> d(64);
+  for (int i = 0; i < 64; ++i) {
+    const int r = i * (S / 64);
+    d[i] = (int64_t)rw.first(r + L) - rw.first(r);
+  }
+  std::sort(d.begin(), d.end());
+  int64_t mode = d[0];
+  int run = 1, best = 1;
+  for (int i = 1; i < 64; ++i) {
+    run = d[i] == d[i - 1] ? run + 1 : 1;
+    if (run > best) { best = run; mode = d[i]; }
+  }
+  if (best < 32) return 0.0;
+  int hit = 0;
+  for (int r = 0; r < S; ++r) hit += ((int64_t)rw.first(r + L) - rw.first(r)) == mode;
+  return (double)hit / S;
+}
+
+// Mean number of 64-byte column sectors (8 columns) that rows r and r + L
+// share, over every 4th sample row.
+static double sector_share(const RowSample &rw, int S, int L) {
+  if (L <= 0 || S + L > rw.n) return 0.0;
+  int64_t tot = 0;
+  for (int r = 0; r < S; r += 4) {
+    const int32_t *a = rw.ci.data() + (rw.rs[r] - rw.rs[0]), *ae = rw.ci.data() + (rw.rs[r + 1] - rw.rs[0]);
+    const int32_t *b = rw.ci.data() + (rw.rs[r + L] - rw.rs[0]), *be = rw.ci.data() + (rw.rs[r + L + 1] - rw.rs[0]);
+    int32_t last = -1;
+    while (a < ae && b < be) {  // members ascend, so sectors do
+      const int32_t x = *a >> 3, y = *b >> 3;
+      if (x < y) ++a;
+      else if (y < x) ++b;
+      else {
+        if (x != last) { ++tot; last = x; }
+        ++a;
+        ++b;
+      }
+    }
+  }
+  return (double)tot / ((S + 3) / 4);
+}
+
+// The shift in [c - w, c + w] (>= lo) that is a lattice shift (shift_freq
+// >= 0.9 of the window's best and >= 0.5) sharing the most sectors; 0: none.
+static int lattice_shift(const RowSample &rw, int S, int c, int w, int lo) {
+  const int a = std::max(lo, c - w), b = c + w;
+  if (a > b) return 0;
+  std::vector<double> fr(b - a + 1);
+  double mx = 0.0;
+  for (int L = a; L <= b; ++L) mx = std::max(mx, fr[L - a] = shift_freq(rw, S, L));
+  int best = 0;
+  double bs = -1.0;
+  for (int L = a; L <= b; ++L) {
+    if (fr[L - a] < 0.5 || fr[L - a] < 0.9 * mx) continue;
+    const double sh = sector_share(rw, S, L);
+    if (sh > bs) { bs = sh; best = L; }
+  }
+  return best;
+}
+
+// Line and layer row strides of the T list (0 where none is found).
+// Candidates: the commonest row distances >= 16 between the rows of
+// adjacent columns over a window of the list's columns; the line stride is
+// the lattice shift nearest the smallest candidate, the layer stride the
+// one within a line of the candidates of at least four lines.
+static int bundle_strides(const Plan *p, hipStream_t st, int32_t *Ry, int32_t *Rz) {
+  *Ry = *Rz = 0;
+  const int32_t w0 = p->Ncols > kBundleWindow ? p->Ncols / 2 - kBundleWindow / 2 : 0;
+  const int32_t wn = std::min<int32_t>(kBundleWindow, p->Ncols - w0);
+  std::vector<int32_t> cr(wn);
+  int rc = (int)hipMemcpyAsync(cr.data(), p->colRowT + w0, sizeof(int32_t) * (size_t)wn, hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) return rc;
+  std::vector<int32_t> d;
+  d.reserve(wn);
+  for (int32_t i = 0; i + 1 < wn; ++i) {
+    if (cr[i] < 0 || cr[i + 1] < 0) continue;
+    const int32_t x = std::abs(cr[i + 1] - cr[i]);
+    if (x >= 16) d.push_back(x);
+  }
+  if (d.empty()) return 0;
+  std::sort(d.begin(), d.end());
+  std::vector<std::pair<int, int32_t>> runs;  // (count, distance)
+  for (size_t i = 0; i < d.size();) {
+    size_t j = i;
+    while (j < d.size() && d[j] == d[i]) ++j;
+    runs.push_back({(int)(j - i), d[i]});
+    i = j;
+  }
+  std::stable_sort(runs.begin(), runs.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+  if (runs.size() > 6) runs.resize(6);
+  int32_t cmin = runs[0].second, cmax = runs[0].second;
+  for (const auto &r : runs) { cmin = std::min(cmin, r.second); cmax = std::max(cmax, r.second); }
+  // rows sampled mid-list, with room for the largest shift tried
+  const int S = kBundleSample;
+  const int64_t need = (int64_t)S + 2 * (int64_t)cmax + 256;
+  if (p->NrowsT < 4 * (int64_t)S || need > p->NrowsT) return 0;
+  RowSample rw;
+  rw.r0 = (int32_t)std::max<int64_t>(0, p->NrowsT / 2 - need / 2);
+  rw.n = (int32_t)std::min<int64_t>(need, p->NrowsT - rw.r0 - 1);
+  rw.rs.resize(rw.n + 1);
+  rc = (int)hipMemcpyAsync(rw.rs.data(), p->rsT + rw.r0, sizeof(int32_t) * (rw.n + 1), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) return rc;
+  rw.ci.resize(rw.rs[rw.n] - rw.rs[0]);
+  rc = (int)hipMemcpyAsync(rw.ci.data(), p->ciT + rw.rs[0], sizeof(int32_t) * rw.ci.size(), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) return rc;
+  const int y = lattice_shift(rw, S, cmin, 64, 16);
+  if (!y) return 0;
+  int z = 0;
+  int tried = 0;
+  for (const auto &r : runs) {  // by count
+    if (r.second < 4 * y || tried == 2) continue;
+    ++tried;
+    const int c = lattice_shift(rw, S, r.second, y + 64, 4 * y);
+    if (c && (!z || sector_share(rw, S, c) > sector_share(rw, S, z))) z = c;
+  }
+  *Ry = y;
+  *Rz = z;
+  return 0;
+}
+
+// Times options 0..n-1 of `launch(i)` against each other: `warm` untimed
+// launches of option 0 first (a memory-bound kernel on this board runs its
+// first launches after a lull faster than its steady state), then `passes`
+// rounds, each timing every option (mean of 8 launches after one untimed)
+// with the starting option rotated; best[i] = option i's fastest round.
+template <class F>
+static int race(F launch, int n, int passes, int warm, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float *best) {
+  for (int i = 0; i < n; ++i) best[i] = 0.f;
+  for (int i = 0; i < warm; ++i) launch(0);
+  int rc = (int)hipGetLastError();
+  for (int pass = 0; pass < passes && !rc; ++pass) {
+    for (int j = 0; j < n && !rc; ++j) {
+      const int c = (pass + j) % n;
+      float us = 0.f;
+      rc = time_launches([&]() { launch(c); }, st, e0, e1, &us);
+      if (!rc && (best[c] == 0.f || us < best[c])) best[c] = us;
+    }
+  }
+  return rc;
+}
+
+static void free_bundle(Bundle &u) {
+  free_delta(u.dp);
+  if (u.part.order) (void)hipFree(u.part.order);
+  if (u.part.d) (void)hipFree(u.part.d);
+  u = Bundle();
+}
+
+// One bundle candidate: its partition, the 32-bit pack its delta pack
+// borrows until the choice is made, and its block order (tiled when that
+// raced 2% faster than block id order).
+struct BundleCand {
+  Bundle u;
+  Pack k;
+  const int *order = nullptr;
+};
+
+static void free_cand(BundleCand &c) {
+  free_bundle(c.u);
+  free_pack(c.k);
+  c = BundleCand();
+}
+
+// Plan::bundle: for a T list of short rows (>= kBundleMinLen entries per row
+// on average) whose partition 0 has a delta pack, the line and layer
+// strides, then bundles of 4 lines x 2 layers and of 4 lines x 1 layer, each
+// cut on entries like partition 0 with a column-sorted pack (a candidate
+// only if every block packs) and a delta pack, tiled where that races 2%
+// faster; the fastest candidate is kept if it races 5% faster than
+// partition 0's launch.  It keeps its 32-bit words for its fallback blocks
+// only.
+static int build_bundle(Plan *p, int32_t *scratch, hipStream_t st) {
+  const Partition &p0 = p->partT[0];
+  if (!p->dpack.w || !p->colRowT || p->nparts == 0 || p0.rowsInKey || p->NrowsT == 0 ||
+      (int64_t)p->nnzT < (int64_t)kBundleMinLen * p->NrowsT || p0.nblocks < 16 * p->cus)
+    return 0;
+  int32_t Ry = 0, Rz = 0;
+  int rc = bundle_strides(p, st, &Ry, &Rz);
+  if (rc || !Ry) return rc;
+  double *vals = nullptr;
+  if (hipMalloc(&vals, sizeof(double) * (size_t)p->Ncols) != hipSuccess) {
+    (void)hipGetLastError();  // no room for scratch values: no bundle
+    return 0;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = (int)hipMemsetAsync(vals, 0, sizeof(double) * (size_t)p->Ncols, st);
+  if (!rc) rc = (int)hipEventCreate(&e0);
+  if (!rc) rc = (int)hipEventCreate(&e1);
+  constexpr int kWarm = 96, kPasses = 5;
+  const int lmax = p->maxRowT;
+  const int shapes[2][2] = {{4, 2}, {4, 1}};
+  BundleCand cand[2];
+  int ncand = 0;
+  for (int c = 0; c < 2 && !rc; ++c) {
+    const int ty = shapes[c][0], tz = shapes[c][1];
+    if (tz > 1 && !Rz) continue;
+    BundleCand &x = cand[ncand];
+    Bundle &u = x.u;
+    u.Ry = Ry;
+    u.Rz = Rz ? Rz : p->NrowsT;
+    u.ty = ty;
+    u.tz = tz;
+    int32_t *rsP = nullptr, *ciP = nullptr;
+    rc = bundle_csr(p->rsT, p->ciT, p->NrowsT, p->nnzT, u.Ry, u.Rz, ty, tz, st, &rsP, &ciP);
+    if (!rc) rc = build_partition(p, rsP, rsP, false, false, p->NrowsT, kCap - (lmax > 0 ? lmax - 1 : 0), scratch, st,
+                                  &u.part);
+    if (!rc) rc = build_pack(u.part, p->NrowsT, rsP, ciP, false, p->nnzT, scratch, st, &x.k);
+    bool ok = !rc && x.k.pk && x.k.plain == 0;
+    if (ok) rc = build_delta_pack(p, u.part, x.k, u.dp, scratch, st);
+    ok = ok && !rc && u.dp.w;
+    int D = 0;
+    if (ok) rc = layer_distance(p, u.part, ciP, st, &D);
+    if (rsP) (void)hipFree(rsP);
+    if (ciP) (void)hipFree(ciP);
+    if (ok && !rc && D) {
+      const std::vector<int> ord = tiled_order(u.part.nblocks, D);
+      rc = (int)hipMalloc(&u.part.order, sizeof(int) * ord.size());
+      if (!rc) rc = (int)hipMemcpyAsync(u.part.order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, st);
+      if (!rc) rc = (int)hipStreamSynchronize(st);
+      float t[2];
+      if (!rc)
+        rc = race([&](int i) {
+          launch_delta_on<double, OGS_ADD>(u.part, u.dp, nullptr, nullptr, p->Ncols, vals, i ? u.part.order : nullptr,
+                                           st);
+        }, 2, kPasses, kWarm, st, e0, e1, t);
+      if (!rc && t[1] < 0.98f * t[0]) {
+        u.part.tileD = D;
+        u.part.tiled = kTiledGS;
+        x.order = u.part.order;
+      }
+    }
+    if (!rc && ok) ++ncand;
+    else free_cand(x);
+  }
+  // partition 0's launch (option 0) against the candidates
+  float t[3] = {0.f, 0.f, 0.f};
+  const int *o0 = (p0.tiled & kTiledGS) ? p0.order : nullptr;
+  if (!rc && ncand)
+    rc = race([&](int i) {
+      if (i == 0) launch_delta_on<double, OGS_ADD>(p0, p->dpack, p->rsT, p->ciT, p->Ncols, vals, o0, st);
+      else launch_delta_on<double, OGS_ADD>(cand[i - 1].u.part, cand[i - 1].u.dp, nullptr, nullptr, p->Ncols, vals,
+                                            cand[i - 1].order, st);
+    }, 1 + ncand, kPasses, kWarm, st, e0, e1, t);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(vals);
+  int pick = -1;
+  for (int i = 0; i < ncand && !rc; ++i)
+    if (t[i + 1] < 0.95f * t[0] && (pick < 0 || t[i + 1] < t[pick + 1])) pick = i;
+  if (pick >= 0) {
+    bool done = false;
+    rc = compact_fallback(cand[pick].u.dp, cand[pick].u.part.nblocks, scratch, st, &done);
+    if (!rc && done) {
+      p->bundle = cand[pick].u;
+      cand[pick].u = Bundle();  // moved into the plan
+    }
+  }
+  for (int i = 0; i < ncand; ++i) free_cand(cand[i]);
+  return rc;
 }
 
 // Device bytes a plan owns (arrays shared between its members counted once).
@@ -3449,9 +3831,19 @@ static int64_t plan_bytes(const Plan *p) {
     part(p->part3[i]);
     pack(p->pack3[i], p->part3[i]);
   }
-  if (p->dpack.w) {
-    add(p->dpack.w, (int64_t)p->partT[0].nblocks * kCap * 2);
-    add(p->dpack.x, (int64_t)p->partT[0].nblocks * kDRanges * p->dpack.E * 4);
+  auto delta = [&](const DeltaPack &d, const Partition &x) {
+    if (!d.w) return;
+    add(d.w, (int64_t)x.nblocks * kCap * 2);
+    add(d.x, (int64_t)x.nblocks * kDRanges * d.E * 4);
+    if (d.ownsF) {
+      add(d.f.pk, (int64_t)d.f.stride * d.fallback * 4);
+      add(d.f.base, (int64_t)d.fallback * (int64_t)sizeof(int4));
+    }
+  };
+  delta(p->dpack, p->partT[0]);
+  if (p->bundle.part.d) {
+    part(p->bundle.part);
+    delta(p->bundle.dp, p->bundle.part);
   }
   add(p->colRowT, (int64_t)p->Ncols * 4);
   add(p->colRowN, (int64_t)p->Ncols * 4);
@@ -3468,6 +3860,7 @@ static void free_plan(Plan *p) {
   if (p->packS.pk && p->packS.pk != p->packT[0].pk) free_pack(p->packS);
   for (int j = 0; j < kPackParts; ++j) free_pack(p->packT[j]);
   free_delta(p->dpack);
+  free_bundle(p->bundle);
   for (int i = 0; i < 2; ++i) {
     free_pack(p->pack3[i]);
     if (p->part3[i].d) (void)hipFree(p->part3[i].d);
@@ -3604,13 +3997,13 @@ int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, i
   for (int j = 0; !rc && j < kMaxParts && NrowsT > 0; ++j) {
     const int W = (kCap >> j) - (lmax > 0 ? lmax - 1 : 0);
     if (W < 1) break;
-    rc = build_partition(p, false, !cutTOnEntries, NrowsT, W, scratch, st, &p->partT[j]);
+    rc = build_partition(p, p->rsT, p->rsN, false, !cutTOnEntries, NrowsT, W, scratch, st, &p->partT[j]);
     if (!rc) {
       if (p->aliased) p->partN[j] = p->partT[j];
-      else rc = build_partition(p, true, true, NrowsN, W, scratch, st, &p->partN[j]);
+      else rc = build_partition(p, p->rsT, p->rsN, true, true, NrowsN, W, scratch, st, &p->partN[j]);
     }
     if (!rc) {
-      if (cutTOnEntries && !p->colRowT) rc = build_partition(p, false, true, NrowsT, W, scratch, st, &p->partS[j]);
+      if (cutTOnEntries && !p->colRowT) rc = build_partition(p, p->rsT, p->rsN, false, true, NrowsT, W, scratch, st, &p->partS[j]);
       else p->partS[j] = p->partT[j];
     }
     p->nparts = j + 1;
@@ -3641,22 +4034,27 @@ int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, i
     for (int i = 0; i < 2 && !rc && cutTOnEntries && !k1; ++i) {
       const int W = kCap3[i] - (lmax > 0 ? lmax - 1 : 0);
       if (W < 1) continue;
-      rc = build_partition(p, false, false, NrowsT, W, scratch, st, &p->part3[i]);
+      rc = build_partition(p, p->rsT, p->rsN, false, false, NrowsT, W, scratch, st, &p->part3[i]);
       if (!rc) rc = build_pack(p->part3[i], NrowsT, p->rsT, p->ciT, false, p->nnzT, scratch, st, &p->pack3[i]);
     }
   }
-  if (!rc && cutTOnEntries) rc = build_delta_pack(p, scratch, st);
-  (void)hipFree(scratch);
+  if (!rc && cutTOnEntries) rc = build_delta_pack(p, p->partT[0], p->packT[0], p->dpack, scratch, st);
   if (!rc && cutTOnEntries) rc = tile_schedule(p, st);
-  // a K = 1 Unsigned plan whose K = 1 kernels all read the delta pack (no
-  // fallback blocks; Scatter by columns) keeps no 32-bit pack of partition 0
-  if (!rc && k1 && p->aliased && jK1 == 0 && p->dpack.w && p->dpack.fallback == 0 && p->colRowT &&
-      p->packT[0].pk) {
-    const uint32_t *gone = p->packT[0].pk;
-    for (Pack *a : {&p->packNT, &p->packNN, &p->packS})
-      if (a->pk == gone) *a = Pack();
-    free_pack(p->packT[0]);
+  if (!rc && cutTOnEntries) rc = build_bundle(p, scratch, st);
+  // a K = 1 Unsigned plan whose K = 1 kernels all read the delta pack
+  // (Scatter by columns) keeps only its fallback blocks' 32-bit words of
+  // partition 0's pack
+  if (!rc && k1 && p->aliased && jK1 == 0 && p->dpack.w && p->colRowT && p->packT[0].pk) {
+    bool done = false;
+    rc = compact_fallback(p->dpack, p->partT[0].nblocks, scratch, st, &done);
+    if (!rc && done) {
+      const uint32_t *gone = p->packT[0].pk;
+      for (Pack *a : {&p->packNT, &p->packNN, &p->packS})
+        if (a->pk == gone) *a = Pack();
+      free_pack(p->packT[0]);
+    }
   }
+  (void)hipFree(scratch);
   if (rc) { free_plan(p); return rc; }
   *plan = p;
   return OGS_SUCCESS;
@@ -3694,6 +4092,13 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->deltaFallback = p->dpack.w ? p->dpack.fallback : 0;
   info->ownerMap = p->colOwner != nullptr;
   info->nMask = p->nMask != nullptr;
+  const Bundle &u = p->bundle;
+  info->bundleRy = u.part.d ? u.Ry : 0;
+  info->bundleRz = u.part.d ? u.Rz : 0;
+  info->bundleTy = u.part.d ? u.ty : 0;
+  info->bundleTz = u.part.d ? u.tz : 0;
+  info->bundleTileD = u.part.d ? u.part.tileD : 0;
+  info->bundleFallback = u.part.d ? u.dp.fallback : 0;
   info->bytes = plan_bytes(p);
   return OGS_SUCCESS;
 }

@@ -210,6 +210,39 @@ int bit_width(uint64_t v) {
   return b;
 }
 
+// Bundle key of row r of a list in first-encounter order (lines of Ry rows,
+// layers of Rz rows): r's bundle of tz layers zb, its line bundle yb within
+// the layer (ty lines each), its position x along the line, its layer l and
+// line m within the bundle.  Keys order bundles, then positions, then layers
+// and lines, so a bundle's rows at one position sit together.
+__global__ void k_bundle_keys(int32_t nrows, int32_t Ry, int32_t Rz, int ty, int tz, uint64_t *__restrict__ key,
+                              int32_t *__restrict__ val) {
+  const int64_t Z = (int64_t)tz * Rz, Y = (int64_t)ty * Ry, nyb = (Rz + Y - 1) / Y;
+  for (int64_t r = blockIdx.x * (int64_t)kSB + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kSB) {
+    const int64_t zb = r / Z, q = r - zb * Z, l = q / Rz, s = q - l * Rz;
+    const int64_t yb = s / Y, t = s - yb * Y, m = t / Ry, x = t - m * Ry;
+    key[r] = (uint64_t)((((zb * nyb + yb) * Ry + x) * tz + l) * ty + m);
+    val[r] = (int32_t)r;
+  }
+}
+
+__global__ void k_bundle_lens(int32_t nrows, const int32_t *__restrict__ rs, const int32_t *__restrict__ perm,
+                              int32_t *__restrict__ len) {
+  for (int64_t i = blockIdx.x * (int64_t)kSB + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * kSB) {
+    const int32_t r = perm[i];
+    len[i] = rs[r + 1] - rs[r];
+  }
+}
+
+__global__ void k_bundle_fill(int32_t nrows, const int32_t *__restrict__ rs, const int32_t *__restrict__ ci,
+                              const int32_t *__restrict__ perm, const int32_t *__restrict__ rsP,
+                              int32_t *__restrict__ ciP) {
+  for (int64_t i = blockIdx.x * (int64_t)kSB + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * kSB) {
+    const int32_t r = perm[i], a = rs[r], n = rs[r + 1] - a, o = rsP[i];
+    for (int32_t k = 0; k < n; ++k) ciP[o + k] = ci[a + k];
+  }
+}
+
 }  // namespace
 
 int device_setup(int32_t N, int64_t *h_ids, int kind, int unique, hipStream_t st, DeviceSetup *out) {
@@ -346,6 +379,50 @@ int device_setup(int32_t N, int64_t *h_ids, int kind, int unique, hipStream_t st
   out->ciN = ciN;
   out->nnzT = (int32_t)M;
   out->nnzN = aliased ? (int32_t)M : nnzN;
+  return OGS_SUCCESS;
+}
+
+int bundle_csr(const int32_t *rs, const int32_t *ci, int32_t nrows, int32_t nnz, int32_t Ry, int32_t Rz, int ty,
+               int tz, hipStream_t st, int32_t **rsP, int32_t **ciP) {
+  *rsP = nullptr;
+  *ciP = nullptr;
+  if (nrows <= 0 || Ry <= 0 || Rz <= 0 || ty <= 0 || tz <= 0) return OGS_E_ARG;
+  Arena A;
+  hipError_t e;
+  uint64_t *k0, *k1;
+  int32_t *v0, *v1, *len;
+  if ((e = A.get(&k0, nrows)) || (e = A.get(&k1, nrows)) || (e = A.get(&v0, nrows)) || (e = A.get(&v1, nrows)) ||
+      (e = A.get(&len, (size_t)nrows + 1)))
+    return (int)e;
+  hipLaunchKernelGGL(k_bundle_keys, dim3(sgrid(nrows)), dim3(kSB), 0, st, nrows, Ry, Rz, ty, tz, k0, v0);
+  // every key lies below (z bundles) * (line bundles per layer) * Ry * tz * ty
+  const int64_t Z = (int64_t)tz * Rz, Y = (int64_t)ty * Ry;
+  const uint64_t bound = (uint64_t)((nrows + Z - 1) / Z) * (uint64_t)((Rz + Y - 1) / Y) * (uint64_t)Ry *
+                         (uint64_t)tz * (uint64_t)ty;
+  const int ebit = bit_width(bound);
+  if ((e = with_temp(A, [&](void *t, size_t &b) {
+         return hipcub::DeviceRadixSort::SortPairs(t, b, k0, k1, v0, v1, nrows, 0, ebit, st);
+       })))
+    return (int)e;
+  hipLaunchKernelGGL(k_bundle_lens, dim3(sgrid(nrows)), dim3(kSB), 0, st, nrows, rs, v1, len);
+  if ((e = hipMemsetAsync(len + nrows, 0, sizeof(int32_t), st))) return (int)e;
+  int32_t *r = nullptr, *c = nullptr;
+  e = hipMalloc(&r, sizeof(int32_t) * ((size_t)nrows + 1));
+  if (e == hipSuccess) e = hipMalloc(&c, sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+  if (e == hipSuccess)
+    e = with_temp(A, [&](void *t, size_t &b) { return hipcub::DeviceScan::ExclusiveSum(t, b, len, r, nrows + 1, st); });
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_bundle_fill, dim3(sgrid(nrows)), dim3(kSB), 0, st, nrows, rs, ci, v1, r, c);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // the arena is freed on return
+  if (e != hipSuccess) {
+    if (r) (void)hipFree(r);
+    if (c) (void)hipFree(c);
+    return (int)e;
+  }
+  *rsP = r;
+  *ciP = c;
   return OGS_SUCCESS;
 }
 

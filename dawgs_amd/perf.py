@@ -79,10 +79,16 @@ def performance_test(nx: int, ny: int, nz: int, p: int, K: int = 1, warmup: int 
     gen = torch.Generator(device="cuda").manual_seed(seed)
     q = torch.empty(N * K, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0, generator=gen).to(dtype)
     gq = torch.zeros(sogs.Ngather * K, dtype=dtype, device="cuda")
+    info = ogs.plan.info()
     res = {"nx": nx, "ny": ny, "nz": nz, "p": p, "K": K, "N": N, "NgatherGlobal": int(ogs.NgatherGlobal),
-           "nnz": ogs.nnzT, "setup_s": setup_s, "tileD": int(ogs.plan.info()["tileD"])}
-    log(f"setup {setup_s:.2f} s (both ogs); GatherScatter block order tiled for a layer distance of "
-        f"{res['tileD']} blocks (0: block id order)")
+           "nnz": ogs.nnzT, "setup_s": setup_s, "tileD": int(info["tileD"]),
+           "bundle": {k: int(info["bundle" + k]) for k in ("Ry", "Rz", "Ty", "Tz", "TileD")}}
+    if info["bundleTy"]:
+        order = (f"bundled blocks of {info['bundleTy']} lines x {info['bundleTz']} layers (row strides "
+                 f"{info['bundleRy']}, {info['bundleRz']}), tiled for a layer distance of {info['bundleTileD']} blocks")
+    else:
+        order = f"block order tiled for a layer distance of {res['tileD']} blocks"
+    log(f"setup {setup_s:.2f} s (both ogs); GatherScatter {order} (0: block id order)")
     t = time_calls(lambda: ogs.GatherScatter(q, K, _ogs.Add, _ogs.Sym), warmup, steps)
     B = gs_bytes(ogs.nnzT, ogs.NrowsT, K, s)
     res["gs"] = {"s": t, "bytes": B, "GBps": B / t / 1e9}

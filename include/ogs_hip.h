@@ -165,12 +165,20 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * each column's owner (GatherScatter NoTrans by columns: a Signed operator
  * with distinct T columns and at most one N member per row, each in its own
  * T row), nMask: 1 where it holds the N-column mask (GatherScatter Trans over
- * the T list).  bytes: device memory the plan owns (partitions, packs,
- * column maps, block orders), beside the caller's CSR. */
+ * the T list).  bundleRy, bundleRz, bundleTy, bundleTz (all 0: none): the
+ * K=1 GatherScatter runs over bundled blocks -- rows of bundleTy
+ * neighbouring lines and bundleTz layers side by side, from the line and
+ * layer row strides bundleRy, bundleRz the plan found in the T list (a
+ * structured mesh's shared nodes at low degree) -- kept where that measured
+ * faster; bundleTileD: their block order's layer distance (0 = block id
+ * order), bundleFallback: their blocks that run from 32-bit words.  bytes:
+ * device memory the plan owns (partitions, packs, column maps, block
+ * orders), beside the caller's CSR. */
 typedef struct {
   int32_t NrowsN, NrowsT, nnzN, nnzT, maxRowT, maxRowN;
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
   int32_t colsT, colsN, deltaE, deltaFallback, ownerMap, nMask;
+  int32_t bundleRy, bundleRz, bundleTy, bundleTz, bundleTileD, bundleFallback;
   int64_t bytes;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);

@@ -259,3 +259,65 @@ def test_k1_plan_signed(torch, D, unique, flag):
         g.GatherScatter(d, 1, D.Add, trans)
         torch.cuda.synchronize()
         assert same(d.cpu().numpy(), o.op_gather_scatter(v.copy(), 1, O.ADD, trans)), trans
+
+
+def test_delta_fallback_blocks_k1_plan(torch, D):
+    """A K = 1 plan keeps only its fallback blocks' 32-bit words (each
+    fallback block finds its words by the slot in its exception words):
+    GatherScatter and Gather bit-exact with fallback blocks mixed in."""
+    ids = shuffled((36, 32, 32, 3), 0.08, 3)
+    g = D.ogs_t()
+    g.Setup(len(ids), ids.copy(), D.Unsigned, D.Auto, False, False, fields=D.PlanK1)
+    info = g.plan.info()
+    assert info["deltaE"] > 0 and 0 < info["deltaFallback"] < info["nblocksK1"] // 4, info
+    assert info["packedK1"] == 0, info  # partition 0's full 32-bit pack is gone
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids))
+    rng = np.random.default_rng(12)
+    for dt, op in ((np.float64, 0), (np.int64, 2), (np.float32, 1)):
+        v = values(rng, dt, len(ids))
+        want = ref.gather_scatter(v.copy(), 1, op, O.SYM)
+        d = torch.from_numpy(v.copy()).cuda()
+        g.GatherScatter(d, 1, op, D.Sym)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), want), (np.dtype(dt).name, op)
+        wg = ref.gather(v, 1, op, O.TRANS)
+        gq = torch.zeros(len(wg), dtype=d.dtype, device="cuda")
+        g.plan.Gather(gq, torch.from_numpy(v.copy()).cuda(), 1, op, D.Trans)
+        torch.cuda.synchronize()
+        assert same(gq.cpu().numpy(), wg), ("gather", np.dtype(dt).name, op)
+
+
+@pytest.mark.parametrize("box,fields", [((192, 192, 192, 1), 1), ((200, 150, 120, 1), 0), ((128, 128, 128, 2), 1)],
+                         ids=["192cubed_p1_k1plan", "200x150x120_p1", "128cubed_p2_k1plan"])
+def test_bundled_blocks(torch, D, box, fields):
+    """Low degree: the plan finds the mesh's line and layer row strides in
+    the T list (first-encounter order), builds GatherScatter blocks of 4
+    lines x 2 or 1 layers and keeps them (they race faster than partition 0
+    at these sizes).  Bit-exact against the oracle for every type and op:
+    each row is still reduced in its own CSR order, whichever block holds
+    it."""
+    ids = O.make_ids(*box)
+    N = len(ids)
+    g = D.ogs_t()
+    g.Setup(N, ids.copy(), D.Unsigned, D.Auto, False, False, fields=fields)
+    info = g.plan.info()
+    nx, ny = box[0], box[1]
+    p = box[3]
+    if p == 1:  # vertex lines and layers of the (nx+1) x (ny+1) x (nz+1) grid
+        assert (info["bundleRy"], info["bundleRz"]) == (nx + 1, (nx + 1) * (ny + 1)), info
+    else:       # element lines: p^3 nx + p^2 rows
+        assert info["bundleRy"] == p ** 3 * nx + p ** 2, info
+    assert info["bundleTy"] == 4 and info["bundleTz"] in (1, 2), info
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=N)
+    del ids
+    rng = np.random.default_rng(sum(box))
+    for dt, op in ((np.float64, 0), (np.float32, 3), (np.int32, 1), (np.int64, 2)):
+        v = values(rng, dt, N)
+        want = ref.gather_scatter(v.copy(), 1, op, O.SYM)
+        d = torch.from_numpy(v.copy()).cuda()
+        g.GatherScatter(d, 1, op, D.Sym)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), want), (box, np.dtype(dt).name, op)
+        del d, want
