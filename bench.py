@@ -47,14 +47,31 @@ def log(*a):
 
 
 def host_cpu() -> str:
+    """lscpu's model, its sockets x cores per socket (the thread count the
+    reference picks when OMP_NUM_THREADS is unset, platformDeviceConfig.cpp:
+    105-146), the CPUs this process may run on, and OMP_NUM_THREADS."""
+    fields = {}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
     except Exception:
         pass
-    return "unknown"
+    model = fields.get("Model name", "unknown")
+    try:
+        rule = int(fields["Socket(s)"]) * int(fields["Core(s) per socket"])
+        topo = f"{fields['Socket(s)']} socket(s) x {fields['Core(s) per socket']} cores"
+    except (KeyError, ValueError):
+        rule, topo = None, "topology unknown"
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return (f"{model} ({topo} = {rule} by the reference's rule; {avail} CPUs available to this process; "
+            f"OMP_NUM_THREADS={omp if omp else 'unset'})")
 
 
 def algorithmic_bytes(nnz: int, nrows: int, K: int, s: int) -> int:

@@ -84,6 +84,7 @@ struct DeltaPack {
   int32_t fallback = 0;                // blocks that run from the 32-bit words
   Pack f;                              // the fallback blocks' 32-bit words
   bool ownsF = false;                  // f is the compacted copy (else the partition's pack)
+  bool rows = false;                   // GatherScatter folds compacted rows (>= 2 entries per row)
 };
 
 // The K = 1 one-list GatherScatter over bundled rows: the T list's rows

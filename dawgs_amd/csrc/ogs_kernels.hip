@@ -962,18 +962,26 @@ __device__ __forceinline__ int head_row(const uint32_t *__restrict__ hm, const i
 // members from CSR position i, VW fields from chunk c, in CSR order from the
 // op's identity, members read four at a time; the result is written back
 // over the row.
-template <typename T, int OP, int KT>
+// LDS slot of CSR position i in a padded stage: one spare slot per 16, so
+// that lanes folding rows of 8 (or 16) members from positions 8 apart read
+// 17 dwords apart and 16 lanes cover the 32 banks once, instead of two.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+constexpr int lds_padded(int n) { return n + n / 16; }
+
+// ... with the stage padded (PAD, KT = 1) or not.
+template <typename T, int OP, int KT, bool PAD = false>
 __device__ __forceinline__ void fold_row(typename Lds<T>::L *__restrict__ val, int i, int c, int len) {
   using L = typename Lds<T>::L;
   using R = LRed<T, OP>;
   constexpr int VW = VecShape<T, KT>::VW;
+  static_assert(!PAD || KT == 1, "padded stages hold one field per entry");
+  auto at = [&](int m, int v) -> L & { return val[(PAD ? lds_pad(i + m) : i + m) * KT + c * VW + v]; };
   typename R::A red[VW];
-  const L *src = val + i * KT + c * VW;
   int mb = 0;
   if constexpr (std::is_same<typename R::A, Comp>::value) {
     // compensated sums take the row's first member (len >= 1) in one step
 #pragma unroll
-    for (int v = 0; v < VW; ++v) red[v] = comp_first(lds_float(src[v]));
+    for (int v = 0; v < VW; ++v) red[v] = comp_first(lds_float(at(0, v)));
     mb = 1;
   } else {
 #pragma unroll
@@ -984,7 +992,7 @@ __device__ __forceinline__ void fold_row(typename Lds<T>::L *__restrict__ val, i
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < VW; ++v) y[u][v] = src[(mb + u) * KT + v];
+      for (int v = 0; v < VW; ++v) y[u][v] = at(mb + u, v);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -992,15 +1000,14 @@ __device__ __forceinline__ void fold_row(typename Lds<T>::L *__restrict__ val, i
   }
   for (; mb < len; ++mb) {
 #pragma unroll
-    for (int v = 0; v < VW; ++v) R::apply(red[v], src[mb * KT + v]);
+    for (int v = 0; v < VW; ++v) R::apply(red[v], at(mb, v));
   }
   L res[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) res[v] = R::fin(red[v]);
-  L *dst = val + i * KT + c * VW;
   for (int r = 0; r < len; ++r)
 #pragma unroll
-    for (int v = 0; v < VW; ++v) dst[r * KT + v] = res[v];
+    for (int v = 0; v < VW; ++v) at(r, v) = res[v];
 }
 
 // LDS of a row-compacted reduction: the head-mask prefix counts and the row
@@ -1253,8 +1260,48 @@ __device__ __forceinline__ void delta_columns(const uint32_t (&w2)[4], uint32_t 
   }
 }
 
+// Row reduction of a delta-pack block staged padded (lds_pad) at its CSR
+// positions: a block of at most kCap / 2 - 1 rows lists its row heads in
+// the dead head bytes and folds one row per lane (at low degree, rows of 4-8
+// members: every lane busy instead of one in eight); a block of more rows
+// (mostly single-member rows) folds at the heads of the entries.  Ends with
+// a barrier.
 template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict__ part,
+__device__ __forceinline__ void delta_reduce(int n, typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                             uint32_t *__restrict__ hm, int32_t *__restrict__ pc) {
+  const int tid = threadIdx.x;
+  head_mask(head, n, hm);
+  __syncthreads();
+  head_prefix(hm, n, pc);
+  __syncthreads();
+  const int last = n - 1, lw = last >> 5;
+  const int rows = pc[lw] + __popc(hm[lw] & (uint32_t)((2ull << (last & 31)) - 1u));
+  if (rows <= kCap / 2 - 1) {
+    uint16_t *rp = reinterpret_cast<uint16_t *>(head);  // head bytes are dead once hm is built
+    for (int q = tid; q < n; q += kBlock) {
+      const uint32_t bits = hm[q >> 5];
+      if ((bits >> (q & 31)) & 1u) rp[pc[q >> 5] + __popc(bits & (uint32_t)((1ull << (q & 31)) - 1u))] = (uint16_t)q;
+    }
+    if (tid == 0) rp[rows] = (uint16_t)n;
+    __syncthreads();
+    for (int r = tid; r < rows; r += kBlock) {
+      const int i = rp[r];
+      fold_row<T, OP, 1, true>(val, i, 0, (int)rp[r + 1] - i);
+    }
+  } else {
+    for (int t = tid; t < n; t += kBlock)
+      if ((hm[t >> 5] >> (t & 31)) & 1u) fold_row<T, OP, 1, true>(val, t, 0, row_len(hm, t));
+  }
+  __syncthreads();
+}
+
+// ROWS: the stage padded (lds_pad) and rows folded by delta_reduce, for
+// lists of at least two entries per row (short blocks of rows; 96^3 p=3
+// 227 -> 219 us); else staged at the plain positions and reduced at the heads
+// of the entries (48^3 p=7: the padded form measured 207 -> 230 us).  At
+// most 64 VGPRs: 8 waves per SIMD.
+template <typename T, int OP, bool ROWS>
+__global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta(const int4 *__restrict__ part,
                                                          const int32_t *__restrict__ starts,
                                                          const int32_t *__restrict__ ids,
                                                          const uint32_t *__restrict__ pk,
@@ -1266,9 +1313,10 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict_
   using V = VecShape<T, 1>;
   static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
   using L = typename Lds<T>::L;
-  __shared__ L val[vec_cap<L>()];
+  __shared__ L val[ROWS ? lds_padded(kCap) : kCap];  // K = 1: at most kCap values, also on the fallback path
   __shared__ uint32_t head4[kCap / 4];
   __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[ROWS ? kCap / 32 : 1];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   const int b = order ? order[xb] : xb;
@@ -1303,17 +1351,21 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(const int4 *__restrict_
     if (i0 + j * 64 < n) {
       const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
       const int ps = (int)(wj & kPackPosMask);
-      val[ps] = Lds<T>::in(x[j]);
+      val[ROWS ? lds_pad(ps) : ps] = Lds<T>::in(x[j]);
       head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
     }
   }
   __syncthreads();
-  vec_reduce<T, OP, 1>(n, val, head, hm);
+  if constexpr (ROWS) {
+    if (n > 0) delta_reduce<T, OP>(n, val, head, hm, pc);
+  } else {
+    vec_reduce<T, OP, 1>(n, val, head, hm);
+  }
 #pragma unroll
   for (int j = 0; j < V::SL; ++j) {
     if (i0 + j * 64 < n) {
       const int ps = (int)((w2[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
-      q[col[j]] = Lds<T>::out(val[ps]);
+      q[col[j]] = Lds<T>::out(val[ROWS ? lds_pad(ps) : ps]);
     }
   }
 }
@@ -2729,9 +2781,14 @@ static int launch_gs_owner(int op, int K, const Plan *p, T *v, hipStream_t st) {
 template <typename T, int OP>
 static void launch_delta_on(const Partition &pt, const DeltaPack &d, const int32_t *rs, const int32_t *ci,
                             int ncols, T *v, const int *order, hipStream_t st) {
-  hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk, d.f.base,
-                     d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
-                     make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), ncols, v, order);
+  const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
+  const u32x4 *dw = reinterpret_cast<const u32x4 *>(d.w);
+  if (d.rows)
+    hipLaunchKernelGGL((k_gs_sym_delta<T, OP, true>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
+                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
+  else
+    hipLaunchKernelGGL((k_gs_sym_delta<T, OP, false>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
+                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
 }
 
 // ... over the plan's bundled blocks where it has them, else partition 0.
@@ -3238,8 +3295,7 @@ static int tile_schedule(Plan *p, hipStream_t st) {
     if (!rc)
       rc = order_pays([&](const int *o) {
         if (dp.w)
-          hipLaunchKernelGGL((k_gs_sym_delta<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
-                             p->ciT, dp.f.pk, dp.f.base, dp.f.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, o);
+          launch_delta_on<double, OGS_ADD>(pt, dp, p->rsT, p->ciT, p->Ncols, vals, o, st);
         else
           hipLaunchKernelGGL((k_gs_sym_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d,
                              p->rsT, p->ciT, k.pk, k.base, k.stride, 0, vals, o);
@@ -3497,6 +3553,7 @@ static int build_delta_pack(const Plan *p, const Partition &pt, const Pack &k, D
   dp.fallback = fb;
   dp.f = k;  // fallback blocks read the partition's pack (slot = block id)
   dp.ownsF = false;
+  dp.rows = (int64_t)p->nnzT >= 2 * (int64_t)p->NrowsT;
   // mostly fallback blocks (columns without short runs): the 32-bit pack serves
   if (!rc && 4 * (int64_t)fb > pt.nblocks) free_delta(dp);
   return rc;
@@ -3597,9 +3654,14 @@ static int lattice_shift(const RowSample &rw, int S, int c, int w, int lo) {
 
 // Line and layer row strides of the T list (0 where none is found).
 // Candidates: the commonest row distances >= 16 between the rows of
-// adjacent columns over a window of the list's columns; the line stride is
-// the lattice shift nearest the smallest candidate, the layer stride the
-// one within a line of the candidates of at least four lines.
+// adjacent columns over a window of the list's columns.  The line stride y
+// is the lattice shift near the smallest candidate sharing the most
+// sectors.  The layer stride is the lattice shift, within a line of the
+// (two commonest) candidates of at least four lines, that maps the line
+// shift's breaks -- the rows whose shift by y differs from the common
+// column shift, at the edges of each layer -- onto themselves: a layer
+// repeats them exactly, a diagonal shift (a line over and a layer up,
+// equally a lattice shift) does not.
 static int bundle_strides(const Plan *p, hipStream_t st, int32_t *Ry, int32_t *Rz) {
   *Ry = *Rz = 0;
   const int32_t w0 = p->Ncols > kBundleWindow ? p->Ncols / 2 - kBundleWindow / 2 : 0;
@@ -3645,15 +3707,37 @@ static int bundle_strides(const Plan *p, hipStream_t st, int32_t *Ry, int32_t *R
   if (rc) return rc;
   const int y = lattice_shift(rw, S, cmin, 64, 16);
   if (!y) return 0;
-  int z = 0;
-  int tried = 0;
+  *Ry = y;
+  // the line shift's common column shift (a sample's mode) and its breaks
+  std::vector<int64_t> dy(S);
+  for (int r = 0; r < S; ++r) dy[r] = (int64_t)rw.first(r + y) - rw.first(r);
+  std::sort(dy.begin(), dy.end());
+  int64_t Dy = dy[0];
+  for (int i = 1, run = 1, best = 1; i < S; ++i) {
+    run = dy[i] == dy[i - 1] ? run + 1 : 1;
+    if (run > best) { best = run; Dy = dy[i]; }
+  }
+  const int nb = rw.n - y;
+  std::vector<uint8_t> brk(nb);
+  for (int r = 0; r < nb; ++r) brk[r] = ((int64_t)rw.first(r + y) - rw.first(r)) != Dy;
+  int z = 0, tried = 0;
+  int64_t zmis = -1;
   for (const auto &r : runs) {  // by count
     if (r.second < 4 * y || tried == 2) continue;
     ++tried;
-    const int c = lattice_shift(rw, S, r.second, y + 64, 4 * y);
-    if (c && (!z || sector_share(rw, S, c) > sector_share(rw, S, z))) z = c;
+    const int a = std::max(4 * y, r.second - y - 64), b = r.second + y + 64;
+    std::vector<double> fr(b - a + 1);
+    double mx = 0.0;
+    for (int L = a; L <= b; ++L) mx = std::max(mx, fr[L - a] = shift_freq(rw, S, L));
+    for (int L = a; L <= b; ++L) {
+      if (fr[L - a] < 0.5 || fr[L - a] < 0.9 * mx) continue;
+      const int m = nb - L;
+      if (m < S) continue;
+      int64_t mis = 0;
+      for (int i = 0; i < m; ++i) mis += brk[i] != brk[i + L];
+      if (zmis < 0 || mis < zmis) { zmis = mis; z = L; }
+    }
   }
-  *Ry = y;
   *Rz = z;
   return 0;
 }
@@ -3705,9 +3789,9 @@ static void free_cand(BundleCand &c) {
 // on average) whose partition 0 has a delta pack, the line and layer
 // strides, then bundles of 4 lines x 2 layers and of 4 lines x 1 layer, each
 // cut on entries like partition 0 with a column-sorted pack (a candidate
-// only if every block packs) and a delta pack, tiled where that races 2%
-// faster; the fastest candidate is kept if it races 5% faster than
-// partition 0's launch.  It keeps its 32-bit words for its fallback blocks
+// only if every block packs) and a delta pack, in the tiled block order for
+// its layer distance; the fastest candidate is kept if it races 5% faster
+// than partition 0's launch.  It keeps its 32-bit words for its fallback blocks
 // only.
 static int build_bundle(Plan *p, int32_t *scratch, hipStream_t st) {
   const Partition &p0 = p->partT[0];
@@ -3752,18 +3836,16 @@ static int build_bundle(Plan *p, int32_t *scratch, hipStream_t st) {
     if (ok) rc = layer_distance(p, u.part, ciP, st, &D);
     if (rsP) (void)hipFree(rsP);
     if (ciP) (void)hipFree(ciP);
+    // tiled whenever the layer distance shows: the element layers between
+    // two bundles of layers are read and written by blocks D apart, and in
+    // block id order half of those lines leave L2 between them (192^3 p=1:
+    // reads 1.41x the minimum, half the 64-byte sectors written in halves)
     if (ok && !rc && D) {
       const std::vector<int> ord = tiled_order(u.part.nblocks, D);
       rc = (int)hipMalloc(&u.part.order, sizeof(int) * ord.size());
       if (!rc) rc = (int)hipMemcpyAsync(u.part.order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, st);
       if (!rc) rc = (int)hipStreamSynchronize(st);
-      float t[2];
-      if (!rc)
-        rc = race([&](int i) {
-          launch_delta_on<double, OGS_ADD>(u.part, u.dp, nullptr, nullptr, p->Ncols, vals, i ? u.part.order : nullptr,
-                                           st);
-        }, 2, kPasses, kWarm, st, e0, e1, t);
-      if (!rc && t[1] < 0.98f * t[0]) {
+      if (!rc) {
         u.part.tileD = D;
         u.part.tiled = kTiledGS;
         x.order = u.part.order;

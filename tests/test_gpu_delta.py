@@ -302,12 +302,14 @@ def test_bundled_blocks(torch, D, box, fields):
     g = D.ogs_t()
     g.Setup(N, ids.copy(), D.Unsigned, D.Auto, False, False, fields=fields)
     info = g.plan.info()
-    nx, ny = box[0], box[1]
-    p = box[3]
-    if p == 1:  # vertex lines and layers of the (nx+1) x (ny+1) x (nz+1) grid
-        assert (info["bundleRy"], info["bundleRz"]) == (nx + 1, (nx + 1) * (ny + 1)), info
-    else:       # element lines: p^3 nx + p^2 rows
-        assert info["bundleRy"] == p ** 3 * nx + p ** 2, info
+    nx, ny, p = box[0], box[1], box[3]
+    # rows each element introduces in first-encounter order: local indices
+    # >= 1 along every axis where a neighbour came before, so an element
+    # line holds (p+1) p^2 + (nx-1) p^3 rows and a layer adds the first
+    # line's (p+1)^2 p + (nx-1) (p+1) p^2
+    Ry = (p + 1) * p ** 2 + (nx - 1) * p ** 3
+    Rz = (p + 1) ** 2 * p + (nx - 1) * (p + 1) * p ** 2 + (ny - 1) * Ry
+    assert (info["bundleRy"], info["bundleRz"]) == (Ry, Rz), info
     assert info["bundleTy"] == 4 and info["bundleTz"] in (1, 2), info
     c = g.csr()
     ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=N)
