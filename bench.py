@@ -161,7 +161,7 @@ def run_sweep(args) -> int:
            "data": "synthetic", "host_threads": O.num_threads() if not args.no_cpu else None,
            "host_cpu": host_cpu() if not args.no_cpu else None,
            "rows": [{"p": r["p"], "nx": r["nx"], "local_dofs": r["N"], "global_dofs": r["NgatherGlobal"],
-                     "gs_tileD": r["tileD"],
+                     "gs_tileD": r["tileD"], "gs_bundle": r["bundle"],
                      "gs_DOFSps": r["NgatherGlobal"] / r["gs"]["s"], "gs_GBps": r["gs"]["GBps"],
                      "gs_frac": r["gs"]["GBps"] / HBM_PEAK_GBS,
                      "gather_DOFSps": r["NgatherGlobal"] / r["gather"]["s"], "gather_GBps": r["gather"]["GBps"],
