@@ -3193,12 +3193,11 @@ static int time_launches(F launch, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
   return rc;
 }
 
-// Whether `order` makes `launch(order)` faster than block id order by the
-// factor `by` (5% unless given): best of three passes each, the candidates'
-// order alternating (the first launches after a setup run slow).
+// Whether `order` makes `launch(order)` at least 5% faster than block id
+// order: best of three passes each, the candidates' order alternating (the
+// first launches after a setup run slow).
 template <class F>
-static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool *pays,
-                      float by = 0.95f) {
+static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool *pays) {
   float best[2] = {0.f, 0.f};
   int rc = 0;
   for (int pass = 0; pass < 3 && !rc; ++pass) {
@@ -3209,7 +3208,7 @@ static int order_pays(F launch, const int *order, hipStream_t st, hipEvent_t e0,
       if (!rc && (best[c] == 0.f || us < best[c])) best[c] = us;
     }
   }
-  *pays = !rc && best[1] < by * best[0];
+  *pays = !rc && best[1] < 0.95f * best[0];
   return rc;
 }
 

@@ -323,3 +323,27 @@ def test_bundled_blocks(torch, D, box, fields):
         torch.cuda.synchronize()
         assert same(d.cpu().numpy(), want), (box, np.dtype(dt).name, op)
         del d, want
+
+
+def test_bundled_blocks_signed_operator(torch, D):
+    """A Signed-unique operator at 192^3 p=1 (the sweep's sogs): its T list
+    gets the same bundled blocks; GatherScatter Sym runs over them bit-exact,
+    NoTrans and Trans (two lists, other kernels) stay bit-exact."""
+    ids = O.make_ids(192, 192, 192, 1)
+    N = len(ids)
+    g = D.ogs_t()
+    sid = ids.copy()
+    g.Setup(N, sid, D.Signed, D.Auto, True, False)
+    del ids
+    info = g.plan.info()
+    assert (info["bundleRy"], info["bundleRz"], info["bundleTy"]) == (193, 193 * 193, 4), info
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], c["rowStartsN"], c["colIdsN"], g.NrowsN, N)
+    rng = np.random.default_rng(193)
+    v = values(rng, np.float64, N)
+    for trans in (O.SYM, O.NOTRANS, O.TRANS):
+        want = ref.gather_scatter(v.copy(), 1, O.ADD, trans)
+        d = torch.from_numpy(v.copy()).cuda()
+        g.GatherScatter(d, 1, D.Add, trans)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), want), trans
