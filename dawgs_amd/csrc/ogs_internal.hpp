@@ -87,6 +87,18 @@ struct DeltaPack {
   bool rows = false;                   // GatherScatter folds compacted rows (>= 2 entries per row)
 };
 
+// A list's column -> row map in 4-bit codes (k_scatter_rows_delta): per
+// 2,048-column chunk 256 code dwords (8 columns each) and 4 E exception
+// words; chunks that do not fit read the int32 map (colRow).
+struct RowMap {
+  uint32_t *codes = nullptr;  // chunks * 256 dwords (null: none)
+  uint32_t *x = nullptr;      // chunks * 4 * E exception words
+  uint32_t *table = nullptr;  // 16 words: the delta of code c (0 at codes 0 and 15)
+  int32_t E = 0;
+  int32_t chunks = 0;
+  int32_t fallback = 0;       // chunks that read colRow
+};
+
 // The K = 1 one-list GatherScatter over bundled rows: the T list's rows
 // reordered so that a block holds the rows of ty neighbouring lines and tz
 // neighbouring layers of a structured mesh side by side (row r's bundle
@@ -144,6 +156,10 @@ struct ogs_hip_plan_s {
   // colRowT for an Unsigned operator
   int32_t *colRowT = nullptr;
   int32_t *colRowN = nullptr;
+  // the same maps in 4-bit row-delta codes, for Scatter at K = 1 (rowMapN
+  // is rowMapT for an Unsigned operator)
+  ogs_hip::RowMap rowMapT;
+  ogs_hip::RowMap rowMapN;
   // per column, the N-list member of its T row (-1: no row; -2: a row with
   // no N member), for operators whose N list holds at most one member per
   // row and whose T list has colRowT (else null): GatherScatter NoTrans K = 1

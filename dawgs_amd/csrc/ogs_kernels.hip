@@ -1710,6 +1710,197 @@ __global__ __launch_bounds__(kBlock) void k_scatter_cols(int32_t ncols, int32_t 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Row-delta map: the column -> row map of a list (colRow) in 4-bit codes,
+// for the by-column Scatter at K = 1.  Workgroup b covers columns
+// [2048 b, 2048 b + 2048) in four wave ranges of 512 (slot j of lane l of
+// wave w: column 2048 b + 512 w + 64 j + l, so each wave instruction covers
+// 64 consecutive columns).  A column's row is the previous row-bearing
+// column's row in its range (0 before the first) plus a delta: code 0 marks
+// a column without a row (delta 0, nothing stored), codes 1-14 add the
+// plan's table entry (the 14 commonest nonzero deltas, 32 bits each: on a
+// structured mesh +1 along an element's new rows, and the jumps back to
+// rows a line or a layer of elements earlier), code 15 adds the range's
+// next exception word (32 bits, wrapping); the
+// first row-bearing column of a range is always an exception (its row, so
+// exception word 0 is a row and never the fallback mark).  A lane's 8 codes
+// make one dword.  Chunks with more than E exceptions in a range carry the
+// fallback mark as every range's exception word 0 and read colRow.
+// 48^3 p=7 (56.6 M columns): 226 MB of colRow per Scatter -> ~0.6 B per
+// column.
+constexpr int kRowChunk = 2048;
+constexpr int kRowSample = 1 << 20;  // columns whose deltas pick the table
+
+// Row of column i and whether it is the first row-bearing column of its
+// range; prev = the previous row-bearing column's row in the range (or 0).
+__device__ __forceinline__ void row_chain(const int32_t *__restrict__ colRow, int64_t ncols, int64_t c0, int i,
+                                          int32_t *row, uint32_t *prev, bool *first) {
+  const int64_t c = c0 + i;
+  *row = c < ncols ? colRow[c] : -1;
+  const int r0 = i & ~(kDWave - 1);
+  int j = i - 1;
+  int32_t pr = -1;
+  while (j >= r0) {
+    const int32_t x = c0 + j < ncols ? colRow[c0 + j] : -1;
+    if (x >= 0) { pr = x; break; }
+    --j;
+  }
+  *first = pr < 0;
+  *prev = pr < 0 ? 0u : (uint32_t)pr;
+}
+
+// Code of a row under the table (entries 1-14 in LDS; 15: not in it).
+__device__ __forceinline__ uint32_t row_code(int32_t row, uint32_t prev, bool first, const uint32_t *tbl) {
+  if (row < 0) return 0u;
+  if (first) return kDEscape;
+  const uint32_t d = (uint32_t)row - prev;
+  uint32_t code = kDEscape;
+#pragma unroll
+  for (int k = 1; k < (int)kDEscape; ++k)
+    if (tbl[k] == d && d != 0u) code = (uint32_t)k;
+  return code;
+}
+
+// Escapes in wave w's range (every lane gets the count).
+__device__ __forceinline__ int row_escapes(const int32_t *__restrict__ colRow, int64_t ncols, int64_t c0,
+                                           const uint32_t *tbl) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int cnt = 0;
+  for (int j = 0; j < kDWave / 64; ++j) {
+    int32_t row;
+    uint32_t prev;
+    bool first;
+    row_chain(colRow, ncols, c0, w * kDWave + j * 64 + lane, &row, &prev, &first);
+    cnt += __builtin_popcountll(__builtin_amdgcn_ballot_w64(row_code(row, prev, first, tbl) == kDEscape));
+  }
+  return cnt;
+}
+
+// hist[min(m, kDMaxE + 1)] += 1 per chunk, m = its most escapes in one range.
+__global__ __launch_bounds__(kBlock) void k_row_delta_count(const int32_t *__restrict__ colRow, int64_t ncols,
+                                                            const uint32_t *__restrict__ table,
+                                                            uint32_t *__restrict__ hist) {
+  __shared__ uint32_t tbl[16];
+  __shared__ int most;
+  if (threadIdx.x < 16) tbl[threadIdx.x] = table[threadIdx.x];
+  if (threadIdx.x == 0) most = 0;
+  __syncthreads();
+  const int c = row_escapes(colRow, ncols, (int64_t)blockIdx.x * kRowChunk, tbl);
+  if ((threadIdx.x & 63) == 0) atomicMax(&most, c);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&hist[most < kDMaxE + 1 ? most : kDMaxE + 1], 1u);
+}
+
+// Codes and exceptions (dx zeroed beforehand); a chunk over E escapes in a
+// range gets the fallback mark and counts in *fallbacks.
+__global__ __launch_bounds__(kBlock) void k_row_delta_build(const int32_t *__restrict__ colRow, int64_t ncols,
+                                                            const uint32_t *__restrict__ table, int E,
+                                                            uint32_t *__restrict__ codes, uint32_t *__restrict__ dx,
+                                                            int32_t *__restrict__ fallbacks) {
+  __shared__ uint32_t tbl[16];
+  __shared__ int over;
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t c0 = (int64_t)b * kRowChunk;
+  if (tid < 16) tbl[tid] = table[tid];
+  if (tid == 0) over = 0;
+  __syncthreads();
+  if (row_escapes(colRow, ncols, c0, tbl) > E && lane == 0) over = 1;
+  __syncthreads();
+  uint32_t *xw = dx + ((size_t)b * kDRanges + w) * E;
+  if (over) {
+    if (lane == 0) xw[0] = kDFallback;
+    if (tid == 0) atomicAdd(fallbacks, 1);
+    return;
+  }
+  uint32_t word = 0u;
+  int run = 0;
+  for (int j = 0; j < kDWave / 64; ++j) {
+    int32_t row;
+    uint32_t prev;
+    bool first;
+    row_chain(colRow, ncols, c0, w * kDWave + j * 64 + lane, &row, &prev, &first);
+    const uint32_t code = row_code(row, prev, first, tbl);
+    const bool esc = code == kDEscape;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (esc) xw[run + lanes_below(mk)] = (uint32_t)row - prev;
+    run += __builtin_popcountll(mk);
+    word |= code << (4 * j);
+  }
+  codes[(size_t)b * kBlock + tid] = word;
+}
+
+// Rows of a lane's 8 slots from its code dword, its range's exception word
+// (lane l holds exception l) and the table (lane k < 16 holds entry k; 0 at
+// codes 0 and 15): per slot the table delta by a lane permute (code 0: 0,
+// no row), an escape by its rank among the range's escapes, an inclusive
+// scan, the running carry.  Rows are clamped to [0, nrows).
+__device__ __forceinline__ uint32_t row_deltas(uint32_t codes, uint32_t ex, uint32_t tv, int nrows,
+                                               int32_t (&row)[kDWave / 64]) {
+  uint32_t carry = 0u, has = 0u;
+  int run = 0;
+#pragma unroll
+  for (int j = 0; j < kDWave / 64; ++j) {
+    const uint32_t code = (codes >> (4 * j)) & 15u;
+    uint32_t d = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(code * 4u), (int)tv);
+    const bool esc = code == kDEscape;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (mk) {
+      const int rank = (run + lanes_below(mk)) & 63;
+      const uint32_t e = (uint32_t)__builtin_amdgcn_ds_bpermute(rank * 4, (int)ex);
+      d = esc ? e : d;
+      run += __builtin_popcountll(mk);
+    }
+    const uint32_t sc = wave_scan_incl(d);
+    const int32_t r = (int32_t)(carry + sc);
+    row[j] = r < 0 ? 0 : (r < nrows ? r : nrows - 1);
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)sc, 63);
+    has |= (code != 0u ? 1u : 0u) << j;
+  }
+  return has;
+}
+
+// Scatter K = 1 by columns over the row-delta map (see k_scatter_cols):
+// one code dword and (first E lanes) one exception word per lane, the rows
+// decoded per wave, their values gathered, then stored in column order.
+// Fallback chunks read colRow.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter_rows_delta(int64_t ncols, int32_t nrows,
+                                                               const int32_t *__restrict__ colRow,
+                                                               const uint32_t *__restrict__ codes,
+                                                               const uint32_t *__restrict__ dx, int E,
+                                                               const uint32_t *__restrict__ table,
+                                                               const T *__restrict__ gv, T *__restrict__ v) {
+  constexpr int U = kDWave / 64;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t c0 = (int64_t)b * kRowChunk + w * kDWave + lane;  // slot j: c0 + 64 j
+  const uint32_t cw = __builtin_nontemporal_load(codes + (size_t)b * kBlock + tid);
+  const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + w) * E + lane) : 0u;
+  const uint32_t tv = lane < 16 ? table[lane] : 0u;
+  int32_t row[U];
+  uint32_t has = 0u;
+  if ((uint32_t)uniform((int)ex) == kDFallback) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int64_t c = c0 + 64 * j;
+      row[j] = c < ncols ? __builtin_nontemporal_load(colRow + c) : -1;
+      has |= (row[j] >= 0 ? 1u : 0u) << j;
+    }
+  } else {
+    has = row_deltas(cw, ex, tv, nrows, row);
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (c0 + 64 * j >= ncols) has &= ~(1u << j);  // (the map codes no row there)
+  }
+  T x[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j)
+    if ((has >> j) & 1u) x[j] = gv[row[j]];
+#pragma unroll
+  for (int j = 0; j < U; ++j)
+    if ((has >> j) & 1u) v[c0 + 64 * j] = x[j];
+}
+
 // colRow[c] = the largest row of rows [0, nrows) whose list holds column c
 // (colRow set to -1 beforehand), one lane per row.
 __global__ __launch_bounds__(kBlock) void k_col_rows(const int32_t *__restrict__ rs, int32_t nrows,
@@ -2950,8 +3141,14 @@ static int launch_scatter_vec(int K, const Partition *vp, const ListView &S, con
 }
 
 template <typename T>
-static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, const T *gv, T *v, hipStream_t st) {
+static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, const T *gv, T *v, hipStream_t st,
+                               const RowMap *rm = nullptr, int32_t nrows = 0) {
   if (ncols == 0) return OGS_SUCCESS;
+  if (K == 1 && rm && rm->codes) {
+    hipLaunchKernelGGL((k_scatter_rows_delta<T>), dim3(rm->chunks), dim3(kBlock), 0, st, (int64_t)ncols, nrows, colRow,
+                       rm->codes, rm->x, rm->E, rm->table, gv, v);
+    return (int)hipGetLastError();
+  }
   const int32_t cpb = K <= kColVals ? kColVals / K : 1;
   const dim3 grid((unsigned)(((int64_t)ncols + cpb - 1) / cpb)), block(kBlock);
   auto go = [&](auto kt) {
@@ -3072,7 +3269,9 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const int32_t *colRow = useN ? p->colRowN : p->colRowT;
   const T *gv = static_cast<const T *>(d_gv);
   T *v = static_cast<T *>(d_v);
-  if (colRow && (int64_t)K * (int64_t)sizeof(T) <= 16) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st);
+  const RowMap *rm = useN ? &p->rowMapN : &p->rowMapT;
+  if (colRow && (int64_t)K * (int64_t)sizeof(T) <= 16)
+    return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st, rm, p->NrowsT);
   const bool partN = useN && !p->aliased;
   // K = 1 from a sparse T list without empty rows: rows staged in LDS, each
   // entry's row found by counting heads, on the merge-path blocks (measured
@@ -3099,7 +3298,7 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
       return launch_scatter_vec<T>(K, vp, list_view(p, false, vp, false), p, gv, v, st);
     }
   }
-  if (colRow) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st);
+  if (colRow) return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st, rm, p->NrowsT);
   const Partition *parts = partN ? p->partN : p->partS;
   int kc = 0;
   const Partition *pt = choose(parts, p->nparts, K, [&](const Partition &x) { return useN ? x.maxN : x.maxT; }, &kc);
@@ -3353,6 +3552,95 @@ static int build_col_rows(const int32_t *rs, int32_t nrows, const int32_t *ci, i
     hipLaunchKernelGGL(k_col_rows, dim3(grid_for(nrows, 16384)), dim3(kBlock), 0, st, rs, nrows, ci, *colRow);
     rc = (int)hipGetLastError();
   }
+  return rc;
+}
+
+static void free_row_map(RowMap &m) {
+  if (m.codes) (void)hipFree(m.codes);
+  if (m.x) (void)hipFree(m.x);
+  if (m.table) (void)hipFree(m.table);
+  m = RowMap();
+}
+
+// The row-delta map of colRow (none when the device has no room or more
+// than a quarter of the chunks would read colRow): the table from a sample
+// of 2^20 columns mid-list (the 14 commonest nonzero deltas between
+// consecutive row-bearing columns), E from a histogram of each chunk's most
+// escapes per range (the fewest map bytes, a fallback chunk counting as its
+// 8 KiB of colRow), then the codes.
+static int build_row_map(const int32_t *colRow, int32_t ncols, int cus, int32_t *scratch, hipStream_t st, RowMap &m) {
+  m = RowMap();
+  if (!colRow || ncols <= 0) return 0;
+  const int chunks = (int)(((int64_t)ncols + kRowChunk - 1) / kRowChunk);
+  if (chunks < kFillPerCu * cus) return 0;  // small lists: the int32 map
+  const int32_t w0 = ncols > kRowSample ? ncols / 2 - kRowSample / 2 : 0;
+  const int32_t wn = std::min<int32_t>(kRowSample, ncols - w0);
+  std::vector<int32_t> cr(wn);
+  int rc = (int)hipMemcpyAsync(cr.data(), colRow + w0, sizeof(int32_t) * (size_t)wn, hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) return rc;
+  std::vector<uint32_t> d;
+  d.reserve(wn);
+  int32_t prev = -1;
+  for (int32_t i = 0; i < wn; ++i) {
+    if (cr[i] < 0) continue;
+    if (prev >= 0 && cr[i] != prev) d.push_back((uint32_t)cr[i] - (uint32_t)prev);
+    prev = cr[i];
+  }
+  std::sort(d.begin(), d.end());
+  std::vector<std::pair<int, uint32_t>> runs;  // (count, delta)
+  for (size_t i = 0; i < d.size();) {
+    size_t j = i;
+    while (j < d.size() && d[j] == d[i]) ++j;
+    runs.push_back({(int)(j - i), d[i]});
+    i = j;
+  }
+  std::stable_sort(runs.begin(), runs.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+  uint32_t tbl[16] = {0u};
+  for (int c = 1; c < (int)kDEscape && c - 1 < (int)runs.size(); ++c) tbl[c] = runs[c - 1].second;
+  uint32_t *hist = nullptr;
+  if (hipMalloc(&m.table, sizeof(tbl)) != hipSuccess || hipMalloc(&hist, sizeof(uint32_t) * (kDMaxE + 2)) != hipSuccess) {
+    (void)hipGetLastError();  // no room: colRow serves
+    free_row_map(m);
+    return 0;
+  }
+  rc = (int)hipMemcpyAsync(m.table, tbl, sizeof(tbl), hipMemcpyHostToDevice, st);
+  if (!rc) rc = (int)hipMemsetAsync(hist, 0, sizeof(uint32_t) * (kDMaxE + 2), st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_row_delta_count, dim3(chunks), dim3(kBlock), 0, st, colRow, (int64_t)ncols, m.table, hist);
+    rc = (int)hipGetLastError();
+  }
+  std::vector<uint32_t> h(kDMaxE + 2, 0u);
+  if (!rc) rc = (int)hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * (kDMaxE + 2), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  (void)hipFree(hist);
+  if (rc) { free_row_map(m); return rc; }
+  double best = -1.0;
+  for (int E = 4; E <= kDMaxE; E += 4) {
+    double over = 0.0;
+    for (int k = E + 1; k <= kDMaxE + 1; ++k) over += h[k];
+    const double bytes = (double)chunks * kDRanges * E * 4.0 + over * 4.0 * kRowChunk;
+    if (best < 0.0 || bytes < best) { best = bytes; m.E = E; }
+  }
+  m.chunks = chunks;
+  const size_t nw = (size_t)chunks * kBlock, nx = (size_t)chunks * kDRanges * m.E;
+  if (hipMalloc(&m.codes, sizeof(uint32_t) * nw) != hipSuccess || hipMalloc(&m.x, sizeof(uint32_t) * nx) != hipSuccess) {
+    (void)hipGetLastError();  // no room: colRow serves
+    free_row_map(m);
+    return 0;
+  }
+  rc = (int)hipMemsetAsync(m.x, 0, sizeof(uint32_t) * nx, st);
+  if (!rc) rc = (int)hipMemsetAsync(scratch, 0, sizeof(int32_t), st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_row_delta_build, dim3(chunks), dim3(kBlock), 0, st, colRow, (int64_t)ncols, m.table, m.E,
+                       m.codes, m.x, scratch);
+    rc = (int)hipGetLastError();
+  }
+  int32_t fb = 0;
+  if (!rc) rc = (int)hipMemcpyAsync(&fb, scratch, sizeof(fb), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  m.fallback = fb;
+  if (rc || 4 * (int64_t)fb > chunks) free_row_map(m);
   return rc;
 }
 
@@ -3926,6 +4214,11 @@ static int64_t plan_bytes(const Plan *p) {
     part(p->bundle.part);
     delta(p->bundle.dp, p->bundle.part);
   }
+  for (const RowMap *m : {&p->rowMapT, &p->rowMapN}) {
+    add(m->codes, (int64_t)m->chunks * kBlock * 4);
+    add(m->x, (int64_t)m->chunks * kDRanges * m->E * 4);
+    add(m->table, 64);
+  }
   add(p->colRowT, (int64_t)p->Ncols * 4);
   add(p->colRowN, (int64_t)p->Ncols * 4);
   add(p->colOwner, (int64_t)p->Ncols * 4);
@@ -3948,6 +4241,8 @@ static void free_plan(Plan *p) {
   }
   if (p->colOwner) (void)hipFree(p->colOwner);
   if (p->nMask) (void)hipFree(p->nMask);
+  if (p->rowMapN.codes && p->rowMapN.codes != p->rowMapT.codes) free_row_map(p->rowMapN);
+  free_row_map(p->rowMapT);
   if (p->colRowN && p->colRowN != p->colRowT) (void)hipFree(p->colRowN);
   if (p->colRowT) (void)hipFree(p->colRowT);
   if (p->partT[0].order) (void)hipFree(p->partT[0].order);
@@ -4048,6 +4343,18 @@ int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, i
     if (p->aliased) p->colRowN = p->colRowT;
     else if (Ncols > 0 && 2 * (int64_t)p->nnzN >= Ncols)
       rc = build_col_rows(p->rsN, NrowsT, p->ciN, Ncols, st, &p->colRowN);
+  }
+  // their row-delta codes, for Scatter at K = 1, on lists of fewer than two
+  // entries per row (mostly single-member rows: the Scatter streams its row
+  // values, and the int32 map was a fifth of its bytes: 48^3 p=7 fp64
+  // 189 -> 165 us, 55^3 p=6 192 -> 183 us; with rows of 4-8 members the
+  // row values are gathered and the decode, in front of those gathers, made
+  // 192^3 p=1 149 -> 167 us)
+  if (!rc && p->colRowT && (int64_t)p->nnzT < 2 * (int64_t)NrowsT)
+    rc = build_row_map(p->colRowT, Ncols, p->cus, scratch, st, p->rowMapT);
+  if (!rc && p->colRowN) {
+    if (p->colRowN == p->colRowT) p->rowMapN = p->rowMapT;
+    else if ((int64_t)p->nnzN < 2 * (int64_t)NrowsN) rc = build_row_map(p->colRowN, Ncols, p->cus, scratch, st, p->rowMapN);
   }
   // the N list as a column mask: GatherScatter Trans over the T list
   if (!rc && !p->aliased && p->colRowT && p->nnzT > 0) rc = build_n_mask(p, NrowsT, scratch, st);
@@ -4180,6 +4487,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->bundleTz = u.part.d ? u.tz : 0;
   info->bundleTileD = u.part.d ? u.part.tileD : 0;
   info->bundleFallback = u.part.d ? u.dp.fallback : 0;
+  info->rowMapE = p->rowMapT.codes ? p->rowMapT.E : 0;
+  info->rowMapFallback = p->rowMapT.codes ? p->rowMapT.fallback : 0;
   info->bytes = plan_bytes(p);
   return OGS_SUCCESS;
 }

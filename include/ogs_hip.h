@@ -171,7 +171,11 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * layer row strides bundleRy, bundleRz the plan found in the T list (a
  * structured mesh's shared nodes at low degree) -- kept where that measured
  * faster; bundleTileD: their block order's layer distance (0 = block id
- * order), bundleFallback: their blocks that run from 32-bit words.  bytes:
+ * order), bundleFallback: their blocks that run from 32-bit words.
+ * rowMapE, rowMapFallback: exception words per 512-column range of the T
+ * list's column -> row map in 4-bit row-delta codes, which Scatter at K=1
+ * by columns reads (0: none, the int32 map serves), and its 2,048-column
+ * chunks that read the int32 map.  bytes:
  * device memory the plan owns (partitions, packs, column maps, block
  * orders), beside the caller's CSR. */
 typedef struct {
@@ -179,6 +183,7 @@ typedef struct {
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
   int32_t colsT, colsN, deltaE, deltaFallback, ownerMap, nMask;
   int32_t bundleRy, bundleRz, bundleTy, bundleTz, bundleTileD, bundleFallback;
+  int32_t rowMapE, rowMapFallback;
   int64_t bytes;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);

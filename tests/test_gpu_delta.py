@@ -347,3 +347,41 @@ def test_bundled_blocks_signed_operator(torch, D):
         g.GatherScatter(d, 1, D.Add, trans)
         torch.cuda.synchronize()
         assert same(d.cpu().numpy(), want), trans
+
+
+@pytest.mark.parametrize("frac", [0.0, 0.08, 1.0], ids=["structured", "partly_shuffled", "shuffled"])
+def test_row_map_scatter(torch, D, frac):
+    """Scatter K = 1 by columns over the column -> row map in 4-bit row-delta
+    codes (lists of fewer than two entries per row, here p = 7): built on a
+    structured list, with fallback chunks (reading the int32 map) where part
+    of the numbering is shuffled, dropped where all of it is; bit-exact
+    against the oracle in every case, for 8- and 4-byte values, NoTrans and
+    Trans of an Unsigned operator."""
+    ids = shuffled((36, 36, 30, 7), frac, 21) if frac else O.make_ids(36, 36, 30, 7)
+    g, _ = setup(D, ids)
+    info = g.plan.info()
+    chunks = -(-len(ids) // 2048)
+    if frac == 0.0:  # (element layers where many rows jump back hold the chunks that fall back: ~11%)
+        assert info["rowMapE"] > 0 and info["rowMapFallback"] < chunks // 6, info
+    elif frac < 1.0:
+        assert info["rowMapE"] > 0 and chunks // 16 < info["rowMapFallback"] <= chunks // 4, info
+    else:
+        assert info["rowMapE"] == 0, info
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids))
+    rng = np.random.default_rng(22)
+    for dt in (np.float64, np.int32):
+        gv = values(rng, dt, g.NrowsT)
+        base = values(rng, dt, len(ids))
+        for trans in (O.NOTRANS, O.TRANS):
+            want = ref.scatter(base.copy(), gv, 1, trans)
+            d = torch.from_numpy(base.copy()).cuda()
+            g.plan.Scatter(d, torch.from_numpy(gv.copy()).cuda(), 1, trans)
+            torch.cuda.synchronize()
+            assert same(d.cpu().numpy(), want), (np.dtype(dt).name, trans)
+
+
+def test_row_map_only_for_short_rows(D):
+    """Lists of two or more entries per row (p = 3) keep the int32 map."""
+    g, _ = setup(D, O.make_ids(36, 32, 32, 3))
+    assert g.plan.info()["rowMapE"] == 0
