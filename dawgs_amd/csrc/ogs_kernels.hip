@@ -1298,28 +1298,19 @@ __device__ __forceinline__ void delta_reduce(int n, typename Lds<T>::L *__restri
 // ROWS: the stage padded (lds_pad) and rows folded by delta_reduce, for
 // lists of at least two entries per row (short blocks of rows; 96^3 p=3
 // 227 -> 219 us); else staged at the plain positions and reduced at the heads
-// of the entries (48^3 p=7: the padded form measured 207 -> 230 us).  At
-// most 64 VGPRs: 8 waves per SIMD.
+// of the entries (48^3 p=7: the padded form measured 207 -> 230 us).
 template <typename T, int OP, bool ROWS>
-__global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta(const int4 *__restrict__ part,
-                                                         const int32_t *__restrict__ starts,
-                                                         const int32_t *__restrict__ ids,
-                                                         const uint32_t *__restrict__ pk,
-                                                         const int4 *__restrict__ base, int stride,
-                                                         const u32x4 *__restrict__ dw,
-                                                         const uint32_t *__restrict__ dx, int E, uint4 tb,
-                                                         int ncols, T *__restrict__ q,
-                                                         const int *__restrict__ order) {
+__device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict__ part,
+                                                   const int32_t *__restrict__ starts,
+                                                   const int32_t *__restrict__ ids, const uint32_t *__restrict__ pk,
+                                                   const int4 *__restrict__ base, int stride,
+                                                   const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx,
+                                                   int E, uint4 tb, int ncols, T *__restrict__ q,
+                                                   typename Lds<T>::L *__restrict__ val, uint32_t *__restrict__ head4,
+                                                   uint32_t *__restrict__ hm, int32_t *__restrict__ pc) {
   using V = VecShape<T, 1>;
   static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
-  using L = typename Lds<T>::L;
-  __shared__ L val[ROWS ? lds_padded(kCap) : kCap];  // K = 1: at most kCap values, also on the fallback path
-  __shared__ uint32_t head4[kCap / 4];
-  __shared__ uint32_t hm[kCap / 32 + 1];
-  __shared__ int32_t pc[ROWS ? kCap / 32 : 1];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
-  const int xb = xcd_block(blockIdx.x, gridDim.x);
-  const int b = order ? order[xb] : xb;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int4 p0 = part[b], p1 = part[b + 1];
   const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
@@ -1369,6 +1360,35 @@ __global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta(const int4 *__restri
     }
   }
 }
+
+#define OGS_DELTA_ARGS                                                                                         \
+  const int4 *__restrict__ part, const int32_t *__restrict__ starts, const int32_t *__restrict__ ids,          \
+      const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride, const u32x4 *__restrict__ dw, \
+      const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_delta(OGS_DELTA_ARGS) {
+  __shared__ typename Lds<T>::L val[kCap];  // K = 1: at most kCap values, also on the fallback path
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  gs_sym_delta_block<T, OP, false>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols,
+                                   q, val, head4, hm, nullptr);
+}
+
+// The ROWS form (lists of >= 2 entries per row), held to 64 VGPRs (8 waves
+// per SIMD; unbounded it took 69).
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta_rows(OGS_DELTA_ARGS) {
+  __shared__ typename Lds<T>::L val[lds_padded(kCap)];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  gs_sym_delta_block<T, OP, true>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols,
+                                  q, val, head4, hm, pc);
+}
+#undef OGS_DELTA_ARGS
 
 // GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
 // the T list alone, on a plan with nMask: each entry's N membership from the
@@ -2975,10 +2995,10 @@ static void launch_delta_on(const Partition &pt, const DeltaPack &d, const int32
   const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
   const u32x4 *dw = reinterpret_cast<const u32x4 *>(d.w);
   if (d.rows)
-    hipLaunchKernelGGL((k_gs_sym_delta<T, OP, true>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
+    hipLaunchKernelGGL((k_gs_sym_delta_rows<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
                        d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
   else
-    hipLaunchKernelGGL((k_gs_sym_delta<T, OP, false>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
+    hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
                        d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
 }
 
