@@ -629,7 +629,8 @@ def main() -> int:
     info = ogs.plan.info()
     log(f"setup {time.perf_counter() - t0:.2f} s (incl. timing): N={N} NgatherGlobal={ogs.NgatherGlobal} "
         f"nnz={ogs.nnzT} blocks={info['nblocksK1']} W={info['widthK1']} maxRow={info['maxRowT']} "
-        f"deltaE={info['deltaE']} deltaFallback={info['deltaFallback']} planMB={info['bytes'] / 1e6:.0f}")
+        f"deltaE={info['deltaE']} deltaFallback={info['deltaFallback']} byteItems={info['byteItems']} "
+        f"byteFallback={info['byteFallback']} planMB={info['bytes'] / 1e6:.0f}")
 
     B = algorithmic_bytes(ogs.nnzT, ogs.NrowsT, 1, 8)
     achieved = B / (ms * 1e-3) / 1e9
@@ -679,7 +680,8 @@ def main() -> int:
                                f"values uniform[-1,1]",
                    "nx": nx, "ny": ny, "nz": nz, "p": p, "K": 1, "op": "Add", "trans": "Sym",
                    "local_dofs": N, "global_dofs": int(ogs.NgatherGlobal), "nnz": ogs.nnzT,
-                   "plan_bytes": info["bytes"], "delta_pack_E": info["deltaE"]},
+                   "plan_bytes": info["bytes"], "delta_pack_E": info["deltaE"],
+                   "byte_codes": {"items": info["byteItems"], "fallback": info["byteFallback"]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -21,7 +21,7 @@ class PlanInfo(C.Structure):
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
                  "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN", "deltaE", "deltaFallback",
                  "ownerMap", "nMask", "bundleRy", "bundleRz", "bundleTy", "bundleTz", "bundleTileD",
-                 "bundleFallback", "rowMapE", "rowMapFallback")] + \
+                 "bundleFallback", "rowMapE", "rowMapFallback", "byteItems", "byteFallback")] + \
                [("bytes", C.c_int64)]
 
 

@@ -85,6 +85,16 @@ struct DeltaPack {
   Pack f;                              // the fallback blocks' 32-bit words
   bool ownsF = false;                  // f is the compacted copy (else the partition's pack)
   bool rows = false;                   // GatherScatter folds compacted rows (>= 2 entries per row)
+  bool seg8 = false;                   // no row has more than 8 members (each thread folds 8 positions)
+  // one-byte codes over the same blocks (k_gs_sym_byte; bw null: none): a
+  // code per entry into a 255-entry table of (column delta, position delta,
+  // head bit), code 255 taking the range's next exception item
+  uint8_t *bw = nullptr;               // nblocks * kCap codes
+  uint32_t *bh = nullptr;              // nblocks * 4 range headers (exception offset << 7 | count)
+  uint32_t *bx = nullptr;              // exception items, two words each
+  uint32_t *bt = nullptr;              // 256 table words
+  int64_t bItems = 0;                  // exception items
+  int32_t bFallback = 0;               // blocks that run from the 16-bit words
 };
 
 // A list's column -> row map in 4-bit codes (k_scatter_rows_delta): per
@@ -97,6 +107,9 @@ struct RowMap {
   int32_t E = 0;
   int32_t chunks = 0;
   int32_t fallback = 0;       // chunks that read colRow
+  // the fallback chunks' rows alone (non-null once compacted: colRow is then
+  // not kept, and a fallback chunk's exception word 1 holds its slot here)
+  int32_t *fbRows = nullptr;
 };
 
 // The K = 1 one-list GatherScatter over bundled rows: the T list's rows

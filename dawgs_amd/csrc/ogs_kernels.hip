@@ -1295,6 +1295,133 @@ __device__ __forceinline__ void delta_reduce(int n, typename Lds<T>::L *__restri
   __syncthreads();
 }
 
+// Types whose row reduction may run as seg8_reduce (an accumulator of the
+// staged type: the reference's four).
+template <typename T>
+constexpr bool kSeg8 = std::is_same<T, double>::value || std::is_same<T, float>::value ||
+                       std::is_same<T, int>::value || std::is_same<T, long long>::value;
+
+// Bits 0..7 of the head bytes held in x (bit k: byte k is 1).  Each 0/1
+// byte b_k of a word lands on bit 24 + k of the product with 0x01020408,
+// the other partial products on distinct lower bits.
+__device__ __forceinline__ uint32_t head_bits8(uint2 x) {
+  const uint32_t lo = ((x.x & 0x01010101u) * 0x01020408u) >> 24;
+  const uint32_t hi = ((x.y & 0x01010101u) * 0x01020408u) >> 24;
+  return lo | (hi << 4);
+}
+
+// Row reduction of a block staged at its CSR positions (not padded) whose
+// rows have at most 8 members.  Thread t owns positions [8t, 8t + 8); every
+// row touching them lies within [8t - 8, 8t + 16), so the thread reads its
+// own values and head bytes and its two neighbours' (before a barrier), and
+// folds in CSR order from the operator's identity: the row open at 8t
+// (begun among the previous thread's positions), its own rows, and its last
+// row on into the next thread's positions up to their first head.  Each own
+// position then takes its row's result.  Positions from n on count as heads.
+// Ends with a barrier.
+template <typename T, int OP>
+__device__ __forceinline__ void seg8_reduce(int n, T *__restrict__ val, const uint8_t *__restrict__ head) {
+  using R = LRed<T, OP>;
+  using A = typename R::A;
+  const int P = 8 * (int)threadIdx.x;
+  const int pp = P >= 8 ? P - 8 : P, pn = P + 8 < kCap ? P + 8 : P;
+  const uint2 *h2 = reinterpret_cast<const uint2 *>(head);
+  uint32_t bits = head_bits8(h2[pp >> 3]) | (head_bits8(h2[P >> 3]) << 8) | (head_bits8(h2[pn >> 3]) << 16);
+  if (P < 8) bits |= 0xffu;  // no previous positions
+  const int live = n - (P - 8);  // bit k is position P - 8 + k: from bit `live` on, past the block's entries
+  if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
+  // three phases, each reading 8 values: held apart (sched_barrier) so that
+  // no more than 16 are in registers at once
+  T x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = val[pp + k];
+  // the row open at P: from the previous positions' last head
+  const int lh = 31 - __builtin_clz(bits & 0xffu);
+  A c = R::init();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    A y = c;
+    R::apply(y, x[k]);
+    c = k >= lh ? y : c;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = val[P + k];
+  A part[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((bits >> (8 + k)) & 1u) c = R::init();
+    R::apply(c, x[k]);
+    part[k] = c;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = val[pn + k];
+  // the last own row on to the next positions' first head
+  const int fh = __builtin_ctz(bits >> 16);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    A y = c;
+    R::apply(y, x[k]);
+    c = k < fh ? y : c;
+  }
+  T res[8];
+  res[7] = R::fin(c);
+#pragma unroll
+  for (int k = 7; k > 0; --k) {
+    if ((bits >> (8 + k)) & 1u) c = part[k - 1];
+    res[k - 1] = R::fin(c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) val[P + k] = res[k];
+  __syncthreads();
+}
+
+// The rest of a delta-coded block once a wave range's columns are known:
+// slot j of the lane holds entry 512 w + 64 j + lane, its CSR position and
+// head bit in half j & 1 of w2[j >> 1] (the 16-bit word's low 12 bits) and
+// its column in col[j].  Loads the values, stages them at their positions,
+// reduces each row in CSR order and stores the results to every member.
+template <typename T, int OP, bool ROWS>
+__device__ __forceinline__ void delta_stage_reduce(const uint32_t (&w2)[4], const uint32_t (&col)[8], int n,
+                                                   T *__restrict__ q, typename Lds<T>::L *__restrict__ val,
+                                                   uint8_t *__restrict__ head, uint32_t *__restrict__ hm,
+                                                   int32_t *__restrict__ pc, bool seg = false) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  T x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = q[col[j]];
+  const int i0 = wv * kDWave + lane;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (i0 + j * 64 < n) {
+      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
+      const int ps = (int)(wj & kPackPosMask);
+      val[ROWS ? lds_pad(ps) : ps] = Lds<T>::in(x[j]);
+      head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
+    }
+  }
+  __syncthreads();
+  if constexpr (ROWS) {
+    if (n > 0) delta_reduce<T, OP>(n, val, head, hm, pc);
+  } else if constexpr (kSeg8<T>) {
+    if (seg) seg8_reduce<T, OP>(n, val, head);
+    else vec_reduce<T, OP, 1>(n, val, head, hm);
+  } else {
+    vec_reduce<T, OP, 1>(n, val, head, hm);
+  }
+  typename Lds<T>::L y[8];  // every position is in the stage: read all, then store
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int ps = (int)((w2[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
+    y[j] = val[ROWS ? lds_pad(ps) : ps];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (i0 + j * 64 < n) q[col[j]] = Lds<T>::out(y[j]);
+}
+
 // ROWS: the stage padded (lds_pad) and rows folded by delta_reduce, for
 // lists of at least two entries per row (short blocks of rows; 96^3 p=3
 // 227 -> 219 us); else staged at the plain positions and reduced at the heads
@@ -1307,7 +1434,8 @@ __device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict
                                                    const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx,
                                                    int E, uint4 tb, int ncols, T *__restrict__ q,
                                                    typename Lds<T>::L *__restrict__ val, uint32_t *__restrict__ head4,
-                                                   uint32_t *__restrict__ hm, int32_t *__restrict__ pc) {
+                                                   uint32_t *__restrict__ hm, int32_t *__restrict__ pc,
+                                                   bool seg = false) {
   using V = VecShape<T, 1>;
   static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
@@ -1329,43 +1457,20 @@ __device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict
     vec_finish<T, OP, 1>(s, m, starts, x, val, head, hm);
     return;
   }
-  const int g0 = uniform(p0.y), n = uniform(p1.y) - g0;
+  const int n = uniform(p1.y) - uniform(p0.y);
   const uint32_t w2[4] = {wd.x, wd.y, wd.z, wd.w};
   uint32_t col[V::SL];
   delta_columns(w2, ex, tb, ncols, col);
-  T x[V::SL];
-#pragma unroll
-  for (int j = 0; j < V::SL; ++j) x[j] = q[col[j]];
-  const int i0 = wv * kDWave + lane;
-#pragma unroll
-  for (int j = 0; j < V::SL; ++j) {
-    if (i0 + j * 64 < n) {
-      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
-      const int ps = (int)(wj & kPackPosMask);
-      val[ROWS ? lds_pad(ps) : ps] = Lds<T>::in(x[j]);
-      head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
-    }
-  }
-  __syncthreads();
-  if constexpr (ROWS) {
-    if (n > 0) delta_reduce<T, OP>(n, val, head, hm, pc);
-  } else {
-    vec_reduce<T, OP, 1>(n, val, head, hm);
-  }
-#pragma unroll
-  for (int j = 0; j < V::SL; ++j) {
-    if (i0 + j * 64 < n) {
-      const int ps = (int)((w2[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
-      q[col[j]] = Lds<T>::out(val[ROWS ? lds_pad(ps) : ps]);
-    }
-  }
+  delta_stage_reduce<T, OP, ROWS>(w2, col, n, q, val, head, hm, pc, seg);
 }
 
 #define OGS_DELTA_ARGS                                                                                         \
   const int4 *__restrict__ part, const int32_t *__restrict__ starts, const int32_t *__restrict__ ids,          \
       const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride, const u32x4 *__restrict__ dw, \
-      const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order
+      const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order, \
+      int seg
 
+// seg: rows of at most 8 members (seg8_reduce)
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(OGS_DELTA_ARGS) {
   __shared__ typename Lds<T>::L val[kCap];  // K = 1: at most kCap values, also on the fallback path
@@ -1373,7 +1478,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(OGS_DELTA_ARGS) {
   __shared__ uint32_t hm[kCap / 32 + 1];
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   gs_sym_delta_block<T, OP, false>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols,
-                                   q, val, head4, hm, nullptr);
+                                   q, val, head4, hm, nullptr, seg != 0);
 }
 
 // The ROWS form (lists of >= 2 entries per row), held to 64 VGPRs (8 waves
@@ -1389,6 +1494,97 @@ __global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta_rows(OGS_DELTA_ARGS)
                                   q, val, head4, hm, pc);
 }
 #undef OGS_DELTA_ARGS
+
+// ---------------------------------------------------------------------------
+// Byte pack: the delta pack's blocks with one byte per entry.
+//
+// An entry's column, CSR position and head bit follow from the previous
+// entry's (in its wave range, column order) by a triple (column delta,
+// position delta mod kCap, head bit).  Code c < 255 names the plan's table
+// word c, (column delta << 12) | (position delta << 1) | head bit, the 255
+// commonest triples with column deltas below 2^20; code 255 takes the range's
+// next exception item, two words (column delta, position delta << 1 | head
+// bit).  The first entry of a range is always an exception (deltas from
+// column 0, position 0).  A block's 2,048 codes are lane-major within each
+// range (lane l's 8 codes contiguous: one 8-byte load per lane); its range
+// headers hold each range's first exception item << 7 | item count (at most
+// 64: one item per lane).  A block with a longer list, or left in CSR
+// order, has kDFallback as header 0 and runs from the 16-bit words.  48^3
+// p=7: 1.05 index bytes per entry instead of 2.13.
+constexpr uint32_t kBEscape = 255u;
+constexpr int kBMaxItems = 64;
+constexpr uint32_t kBColLimit = 1u << 20;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Columns, CSR positions and head bits of a wave's 8 slots from its codes
+// (cw: 4 per word) and its exception item (lane l holds the range's item l):
+// per slot the two deltas (an exception's by its rank among the range's
+// escapes, through lane permutes), two inclusive scans, the running carries.
+// Positions and head bits come out as the 16-bit words' low 12 bits (two per
+// dword of w2); columns are clamped to [0, ncols).
+__device__ __forceinline__ void byte_columns(u32x2 cw, u32x2 ex, const uint32_t *__restrict__ tab, int ncols,
+                                             uint32_t (&col)[8], uint32_t (&w2)[4]) {
+  uint32_t carry = 0u, pcarry = 0u;
+  int run = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t code = ((j < 4 ? cw.x : cw.y) >> ((j & 3) * 8)) & 255u;
+    const uint32_t t = tab[code];
+    uint32_t dc = t >> 12, dp = t & 0xfffu;
+    const bool esc = code == kBEscape;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (mk) {
+      const int rank = (run + lanes_below(mk)) & 63;
+      const uint32_t e0 = (uint32_t)__builtin_amdgcn_ds_bpermute(rank * 4, (int)ex.x);
+      const uint32_t e1 = (uint32_t)__builtin_amdgcn_ds_bpermute(rank * 4, (int)ex.y);
+      dc = esc ? e0 : dc;
+      dp = esc ? e1 : dp;
+      run += __builtin_popcountll(mk);
+    }
+    const uint32_t sc = wave_scan_incl(dc);
+    const uint32_t c = carry + sc;
+    col[j] = c < (uint32_t)ncols ? c : (uint32_t)(ncols - 1);
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)sc, 63);
+    const uint32_t sp = wave_scan_incl(dp >> 1);
+    const uint32_t wj = ((pcarry + sp) & kPackPosMask) | ((dp & 1u) << kPackHeadBit);
+    pcarry += (uint32_t)__builtin_amdgcn_readlane((int)sp, 63);
+    if (j & 1) w2[j >> 1] |= wj << 16;
+    else w2[j >> 1] = wj;
+  }
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_byte(
+    const int4 *__restrict__ part, const int32_t *__restrict__ starts, const int32_t *__restrict__ ids,
+    const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride, const u32x4 *__restrict__ dw,
+    const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order,
+    int seg, const u32x2 *__restrict__ bw, const uint4 *__restrict__ bh, const u32x2 *__restrict__ bx,
+    const uint32_t *__restrict__ bt) {
+  __shared__ typename Lds<T>::L val[kCap];
+  __shared__ uint32_t head4[kCap / 4];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ uint32_t tab[256];
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const uint4 h = bh[b];
+  if (h.x == kDFallback) {
+    gs_sym_delta_block<T, OP, false>(b, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols, q, val, head4, hm,
+                                     nullptr, seg != 0);
+    return;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const u32x2 cw = __builtin_nontemporal_load(bw + (size_t)b * kBlock + tid);
+  tab[tid] = bt[tid];
+  const uint32_t hr = (uint32_t)uniform((int)(wv == 0 ? h.x : wv == 1 ? h.y : wv == 2 ? h.z : h.w));
+  const int cnt = (int)(hr & 127u);
+  u32x2 ex = {0u, 0u};
+  if (lane < cnt) ex = __builtin_nontemporal_load(bx + (size_t)(hr >> 7) + lane);
+  const int n = uniform(part[b + 1].y) - uniform(part[b].y);
+  __syncthreads();
+  uint32_t col[8], w2[4];
+  byte_columns(cw, ex, tab, ncols, col, w2);
+  delta_stage_reduce<T, OP, false>(w2, col, n, q, val, reinterpret_cast<uint8_t *>(head4), hm, nullptr, seg != 0);
+}
 
 // GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
 // the T list alone, on a plan with nMask: each entry's N membership from the
@@ -1882,10 +2078,12 @@ __device__ __forceinline__ uint32_t row_deltas(uint32_t codes, uint32_t ex, uint
 // Scatter K = 1 by columns over the row-delta map (see k_scatter_cols):
 // one code dword and (first E lanes) one exception word per lane, the rows
 // decoded per wave, their values gathered, then stored in column order.
-// Fallback chunks read colRow.
+// Fallback chunks read colRow, or, with fbRows, their slot there (exception
+// word 1).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter_rows_delta(int64_t ncols, int32_t nrows,
                                                                const int32_t *__restrict__ colRow,
+                                                               const int32_t *__restrict__ fbRows,
                                                                const uint32_t *__restrict__ codes,
                                                                const uint32_t *__restrict__ dx, int E,
                                                                const uint32_t *__restrict__ table,
@@ -1900,10 +2098,13 @@ __global__ __launch_bounds__(kBlock) void k_scatter_rows_delta(int64_t ncols, in
   int32_t row[U];
   uint32_t has = 0u;
   if ((uint32_t)uniform((int)ex) == kDFallback) {
+    const int64_t slot = __builtin_amdgcn_readlane((int)ex, 1);
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       const int64_t c = c0 + 64 * j;
-      row[j] = c < ncols ? __builtin_nontemporal_load(colRow + c) : -1;
+      if (c >= ncols) row[j] = -1;
+      else if (fbRows) row[j] = __builtin_nontemporal_load(fbRows + slot * kRowChunk + (c - (int64_t)b * kRowChunk));
+      else row[j] = __builtin_nontemporal_load(colRow + c);
       has |= (row[j] >= 0 ? 1u : 0u) << j;
     }
   } else {
@@ -1919,6 +2120,25 @@ __global__ __launch_bounds__(kBlock) void k_scatter_rows_delta(int64_t ncols, in
 #pragma unroll
   for (int j = 0; j < U; ++j)
     if ((has >> j) & 1u) v[c0 + 64 * j] = x[j];
+}
+
+// The fallback chunks' rows of a row-delta map in an array of their own:
+// each fallback chunk takes the next slot (written as its ranges' exception
+// word 1) and copies its kRowChunk entries of colRow there (-1 past ncols).
+__global__ __launch_bounds__(kBlock) void k_compact_rows(uint32_t *__restrict__ dx, int E,
+                                                         const int32_t *__restrict__ colRow, int64_t ncols,
+                                                         int32_t *__restrict__ fbRows, int32_t *__restrict__ count) {
+  __shared__ int slot;
+  const int b = blockIdx.x;
+  if (dx[(size_t)b * kDRanges * E] != kDFallback) return;
+  if (threadIdx.x == 0) slot = atomicAdd(count, 1);
+  __syncthreads();
+  const int sl = slot;
+  if (threadIdx.x < kDRanges) dx[((size_t)b * kDRanges + threadIdx.x) * E + 1] = (uint32_t)sl;
+  for (int i = threadIdx.x; i < kRowChunk; i += kBlock) {
+    const int64_t c = (int64_t)b * kRowChunk + i;
+    fbRows[(size_t)sl * kRowChunk + i] = c < ncols ? colRow[c] : -1;
+  }
 }
 
 // colRow[c] = the largest row of rows [0, nrows) whose list holds column c
@@ -2712,6 +2932,132 @@ __global__ __launch_bounds__(kBlock) void k_compact_fallback(uint32_t *__restric
   for (int i = threadIdx.x; i < stride; i += kBlock) fpk[(size_t)sl * stride + i] = pk[(size_t)b * stride + i];
 }
 
+// Byte-pack builders, one workgroup per block of partition 0, from the
+// block's 32-bit pack words.  The triple key of sorted entry i (i > 0 in its
+// wave range): (column delta << 12) | (position delta << 1) | head bit, or
+// kDFallback where the column delta is not in [0, 2^20).
+__device__ __forceinline__ uint32_t byte_key(const uint32_t *__restrict__ pk, const int4 &bb, int stride, int b,
+                                             int i) {
+  const uint32_t w = pk[(size_t)b * stride + i], v = pk[(size_t)b * stride + i - 1];
+  const uint32_t d = (uint32_t)(slot_col(w, bb, true) - slot_col(v, bb, true));
+  if (d >= kBColLimit) return kDFallback;
+  const uint32_t dp = ((w & kPackPosMask) - (v & kPackPosMask)) & kPackPosMask;
+  return (d << 12) | (dp << 1) | ((w >> kPackHeadBit) & 1u);
+}
+
+// keys[s * kCap + i]: the keys of sample block s * every (kDFallback where
+// there is none), the candidates of the byte table.
+__global__ __launch_bounds__(kBlock) void k_byte_keys(const int4 *__restrict__ part, const uint32_t *__restrict__ pk,
+                                                      const int4 *__restrict__ base, int stride, int every,
+                                                      uint32_t *__restrict__ keys) {
+  const int s = blockIdx.x, b = s * every;
+  const int4 bb = base[b];
+  const int n = part[b + 1].y - part[b].y;
+  for (int i = threadIdx.x; i < kCap; i += kBlock)
+    keys[(size_t)s * kCap + i] = (bb.x >= 0 && i < n && i % kDWave != 0) ? byte_key(pk, bb, stride, b, i) : kDFallback;
+}
+
+// Code of entry i of a block under the sorted table keys sk (code sc[k] for
+// key sk[k], nk keys; kBEscape when absent), and the exception item it
+// would need.
+__device__ __forceinline__ uint32_t byte_code(const uint32_t *__restrict__ pk, const int4 &bb, int stride, int b,
+                                              int i, const uint32_t *sk, const uint8_t *sc, int nk, u32x2 &item) {
+  const uint32_t w = pk[(size_t)b * stride + i];
+  const uint32_t c = (uint32_t)slot_col(w, bb, true);
+  uint32_t pc = 0u, pp = 0u;
+  if (i % kDWave != 0) {
+    const uint32_t v = pk[(size_t)b * stride + i - 1];
+    pc = (uint32_t)slot_col(v, bb, true);
+    pp = v & kPackPosMask;
+  }
+  item.x = c - pc;
+  item.y = ((((w & kPackPosMask) - pp) & kPackPosMask) << 1) | ((w >> kPackHeadBit) & 1u);
+  if (i % kDWave == 0 || item.x >= kBColLimit) return kBEscape;
+  const uint32_t key = (item.x << 12) | item.y;
+  int lo = 0, hi = nk;  // first sk[k] >= key
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sk[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < nk && sk[lo] == key) ? (uint32_t)sc[lo] : kBEscape;
+}
+
+__device__ __forceinline__ void byte_table(const uint32_t *__restrict__ skey, const uint8_t *__restrict__ scode,
+                                           uint32_t *sk, uint8_t *sc) {
+  sk[threadIdx.x] = skey[threadIdx.x];
+  sc[threadIdx.x] = scode[threadIdx.x];
+  __syncthreads();
+}
+
+// cnt[4 b + w]: the exception items of wave range w of block b, or 0 for
+// every range of a block that does not fit (left in CSR order, or a range
+// of more than kBMaxItems items).
+__global__ __launch_bounds__(kBlock) void k_byte_count(const int4 *__restrict__ part, const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       const uint32_t *__restrict__ skey,
+                                                       const uint8_t *__restrict__ scode, int nk,
+                                                       uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t sk[256];
+  __shared__ uint8_t sc[256];
+  __shared__ int over;
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  byte_table(skey, scode, sk, sc);
+  if (tid == 0) over = 0;
+  const int4 bb = base[b];
+  const int n = part[b + 1].y - part[b].y;
+  int c = 0;
+  if (bb.x >= 0) {
+    for (int j = 0; j < kDWave / 64; ++j) {
+      const int i = w * kDWave + j * 64 + lane;
+      u32x2 item;
+      const bool esc = i < n && byte_code(pk, bb, stride, b, i, sk, sc, nk, item) == kBEscape;
+      c += __builtin_popcountll(__builtin_amdgcn_ballot_w64(esc));
+    }
+  }
+  __syncthreads();
+  if (lane == 0 && (bb.x < 0 || c > kBMaxItems)) over = 1;
+  __syncthreads();
+  if (lane == 0) cnt[(size_t)b * kDRanges + w] = over ? 0u : (uint32_t)c;
+}
+
+// The byte pack's codes, headers and exception items (off: each range's
+// first item, the exclusive sum of k_byte_count's counts).
+__global__ __launch_bounds__(kBlock) void k_byte_build(const int4 *__restrict__ part, const uint32_t *__restrict__ pk,
+                                                       const int4 *__restrict__ base, int stride,
+                                                       const uint32_t *__restrict__ skey,
+                                                       const uint8_t *__restrict__ scode, int nk,
+                                                       const uint32_t *__restrict__ cnt,
+                                                       const uint32_t *__restrict__ off, uint8_t *__restrict__ bw,
+                                                       uint32_t *__restrict__ bh, u32x2 *__restrict__ bx) {
+  __shared__ uint32_t sk[256];
+  __shared__ uint8_t sc[256];
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  byte_table(skey, scode, sk, sc);
+  const int4 bb = base[b];
+  const int n = part[b + 1].y - part[b].y;
+  bool fits = bb.x >= 0;
+  for (int r = 0; r < kDRanges && fits; ++r) fits = cnt[(size_t)b * kDRanges + r] > 0u || n <= r * kDWave;
+  if (!fits) {
+    if (tid < kDRanges) bh[(size_t)b * kDRanges + tid] = tid == 0 ? kDFallback : 0u;
+    return;
+  }
+  const uint32_t o = off[(size_t)b * kDRanges + w];
+  if (lane == 0) bh[(size_t)b * kDRanges + w] = (o << 7) | cnt[(size_t)b * kDRanges + w];
+  int run = 0;
+  for (int j = 0; j < kDWave / 64; ++j) {
+    const int i = w * kDWave + j * 64 + lane;
+    uint32_t code = 0u;
+    u32x2 item = {0u, 0u};
+    if (i < n) code = byte_code(pk, bb, stride, b, i, sk, sc, nk, item);
+    const bool esc = i < n && code == kBEscape;
+    const uint64_t mk = __builtin_amdgcn_ballot_w64(esc);
+    if (esc) bx[o + run + lanes_below(mk)] = item;
+    run += __builtin_popcountll(mk);
+    bw[(size_t)b * kCap + w * kDWave + lane * (kDWave / 64) + j] = (uint8_t)code;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers.
 
@@ -2994,12 +3340,17 @@ static void launch_delta_on(const Partition &pt, const DeltaPack &d, const int32
                             int ncols, T *v, const int *order, hipStream_t st) {
   const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
   const u32x4 *dw = reinterpret_cast<const u32x4 *>(d.w);
-  if (d.rows)
+  const int seg = d.seg8 ? 1 : 0;
+  if (d.bw)
+    hipLaunchKernelGGL((k_gs_sym_byte<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk, d.f.base,
+                       d.f.stride, dw, d.x, d.E, tb, ncols, v, order, seg, reinterpret_cast<const u32x2 *>(d.bw),
+                       reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
+  else if (d.rows)
     hipLaunchKernelGGL((k_gs_sym_delta_rows<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
-                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
+                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order, 0);
   else
     hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
-                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
+                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order, seg);
 }
 
 // ... over the plan's bundled blocks where it has them, else partition 0.
@@ -3166,7 +3517,7 @@ static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, cons
   if (ncols == 0) return OGS_SUCCESS;
   if (K == 1 && rm && rm->codes) {
     hipLaunchKernelGGL((k_scatter_rows_delta<T>), dim3(rm->chunks), dim3(kBlock), 0, st, (int64_t)ncols, nrows, colRow,
-                       rm->codes, rm->x, rm->E, rm->table, gv, v);
+                       rm->fbRows, rm->codes, rm->x, rm->E, rm->table, gv, v);
     return (int)hipGetLastError();
   }
   const int32_t cpb = K <= kColVals ? kColVals / K : 1;
@@ -3290,7 +3641,7 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const T *gv = static_cast<const T *>(d_gv);
   T *v = static_cast<T *>(d_v);
   const RowMap *rm = useN ? &p->rowMapN : &p->rowMapT;
-  if (colRow && (int64_t)K * (int64_t)sizeof(T) <= 16)
+  if ((colRow || (K == 1 && rm->codes)) && (int64_t)K * (int64_t)sizeof(T) <= 16)
     return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st, rm, p->NrowsT);
   const bool partN = useN && !p->aliased;
   // K = 1 from a sparse T list without empty rows: rows staged in LDS, each
@@ -3576,6 +3927,7 @@ static int build_col_rows(const int32_t *rs, int32_t nrows, const int32_t *ci, i
 }
 
 static void free_row_map(RowMap &m) {
+  if (m.fbRows) (void)hipFree(m.fbRows);
   if (m.codes) (void)hipFree(m.codes);
   if (m.x) (void)hipFree(m.x);
   if (m.table) (void)hipFree(m.table);
@@ -3664,6 +4016,34 @@ static int build_row_map(const int32_t *colRow, int32_t ncols, int cus, int32_t 
   return rc;
 }
 
+// Moves the rows of row map m's fallback chunks out of colRow (ncols
+// columns) into an array of their own, so that colRow can go.  False (m
+// unchanged) when the device has no room.
+static int compact_rows(RowMap &m, const int32_t *colRow, int32_t ncols, int32_t *scratch, hipStream_t st,
+                        bool *done) {
+  *done = false;
+  if (!m.codes || m.fbRows) return 0;
+  int32_t *fb = nullptr;
+  if (m.fallback > 0 && hipMalloc(&fb, sizeof(int32_t) * (size_t)m.fallback * kRowChunk) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int rc = (int)hipMemsetAsync(scratch, 0, sizeof(int32_t), st);
+  if (!rc && fb) {
+    hipLaunchKernelGGL(k_compact_rows, dim3(m.chunks), dim3(kBlock), 0, st, m.x, m.E, colRow, (int64_t)ncols, fb,
+                       scratch);
+    rc = (int)hipGetLastError();
+  }
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) {
+    if (fb) (void)hipFree(fb);
+    return rc;
+  }
+  m.fbRows = fb;  // (no fallback chunk: nothing reads colRow)
+  *done = true;
+  return 0;
+}
+
 // nMask of a plan with colRowT (no room or a failed check: none).
 static int build_n_mask(Plan *p, int32_t nrows, int32_t *scratch, hipStream_t st) {
   const size_t words = ((size_t)p->Ncols + 31) / 32;
@@ -3747,7 +4127,17 @@ static void free_pack(Pack &k) {
   k = Pack();
 }
 
+static void free_byte(DeltaPack &d) {
+  for (void *a : {(void *)d.bw, (void *)d.bh, (void *)d.bx, (void *)d.bt})
+    if (a) (void)hipFree(a);
+  d.bw = nullptr;
+  d.bh = d.bx = d.bt = nullptr;
+  d.bItems = 0;
+  d.bFallback = 0;
+}
+
 static void free_delta(DeltaPack &d) {
+  free_byte(d);
   if (d.w) (void)hipFree(d.w);
   if (d.x) (void)hipFree(d.x);
   if (d.ownsF) free_pack(d.f);
@@ -3861,9 +4251,114 @@ static int build_delta_pack(const Plan *p, const Partition &pt, const Pack &k, D
   dp.f = k;  // fallback blocks read the partition's pack (slot = block id)
   dp.ownsF = false;
   dp.rows = (int64_t)p->nnzT >= 2 * (int64_t)p->NrowsT;
+  dp.seg8 = p->maxRowT <= 8;
   // mostly fallback blocks (columns without short runs): the 32-bit pack serves
   if (!rc && 4 * (int64_t)fb > pt.nblocks) free_delta(dp);
   return rc;
+}
+
+// The byte pack of delta pack dp's blocks (partition pt, its 32-bit pack k):
+// the table from the triples of up to 512 sample blocks spread over the
+// partition (the 255 commonest), each range's exception items counted, their
+// offsets summed on the host, then the codes.  Built for blocks that fold at
+// the heads of their entries (fewer than 2 entries per row); none when the
+// device has no room or more than a quarter of the blocks would fall back.
+constexpr int kByteSample = 512;
+
+static int build_byte_pack(const Partition &pt, const Pack &k, DeltaPack &dp, hipStream_t st) {
+  if (!dp.w || dp.rows || !k.pk || pt.nblocks == 0) return 0;
+  const int nb = pt.nblocks;
+  const int every = nb > kByteSample ? nb / kByteSample : 1;
+  const int ns = (nb - 1) / every + 1;
+  uint32_t *keys = nullptr, *skey = nullptr, *cnt = nullptr, *off = nullptr;
+  uint8_t *scode = nullptr;
+  auto release = [&]() {
+    for (void *a : {(void *)keys, (void *)skey, (void *)cnt, (void *)off, (void *)scode})
+      if (a) (void)hipFree(a);
+  };
+  const size_t nr = (size_t)nb * kDRanges;
+  if (hipMalloc(&keys, sizeof(uint32_t) * (size_t)ns * kCap) != hipSuccess ||
+      hipMalloc(&skey, sizeof(uint32_t) * 256) != hipSuccess || hipMalloc(&scode, 256) != hipSuccess ||
+      hipMalloc(&cnt, sizeof(uint32_t) * nr) != hipSuccess || hipMalloc(&off, sizeof(uint32_t) * nr) != hipSuccess) {
+    (void)hipGetLastError();  // no room: the 16-bit words serve
+    release();
+    return 0;
+  }
+  hipLaunchKernelGGL(k_byte_keys, dim3(ns), dim3(kBlock), 0, st, pt.d, k.pk, k.base, k.stride, every, keys);
+  int rc = (int)hipGetLastError();
+  std::vector<uint32_t> h((size_t)ns * kCap);
+  if (!rc) rc = (int)hipMemcpyAsync(h.data(), keys, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) { release(); return rc; }
+  std::sort(h.begin(), h.end());
+  std::vector<std::pair<uint32_t, uint32_t>> runs;  // (count, key)
+  for (size_t i = 0; i < h.size() && h[i] != kDFallback;) {
+    size_t j = i;
+    while (j < h.size() && h[j] == h[i]) ++j;
+    runs.push_back({(uint32_t)(j - i), h[i]});
+    i = j;
+  }
+  std::stable_sort(runs.begin(), runs.end(), [](const std::pair<uint32_t, uint32_t> &a,
+                                                const std::pair<uint32_t, uint32_t> &b) { return a.first > b.first; });
+  const int nk = (int)std::min<size_t>(runs.size(), kBEscape);
+  std::vector<uint32_t> tbl(256, 0u), sk(256, kDFallback);
+  std::vector<uint8_t> sc(256, (uint8_t)kBEscape);
+  std::vector<std::pair<uint32_t, uint8_t>> byKey;
+  for (int c = 0; c < nk; ++c) {
+    tbl[c] = runs[c].second;
+    byKey.push_back({runs[c].second, (uint8_t)c});
+  }
+  std::sort(byKey.begin(), byKey.end());
+  for (int i = 0; i < nk; ++i) {
+    sk[i] = byKey[i].first;
+    sc[i] = byKey[i].second;
+  }
+  rc = (int)hipMemcpyAsync(skey, sk.data(), sizeof(uint32_t) * 256, hipMemcpyHostToDevice, st);
+  if (!rc) rc = (int)hipMemcpyAsync(scode, sc.data(), 256, hipMemcpyHostToDevice, st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_byte_count, dim3(nb), dim3(kBlock), 0, st, pt.d, k.pk, k.base, k.stride, skey, scode, nk,
+                       cnt);
+    rc = (int)hipGetLastError();
+  }
+  std::vector<uint32_t> c(nr), o(nr);
+  if (!rc) rc = (int)hipMemcpyAsync(c.data(), cnt, sizeof(uint32_t) * nr, hipMemcpyDeviceToHost, st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  if (rc) { release(); return rc; }
+  int64_t items = 0;
+  int32_t fb = 0;
+  for (size_t r = 0; r < nr; ++r) {
+    o[r] = (uint32_t)items;
+    items += c[r];
+    if (r % kDRanges == 0 && c[r] == 0u) ++fb;
+  }
+  if (items >= (int64_t)1 << 25 || 4 * (int64_t)fb > nb) { release(); return 0; }
+  DeltaPack b;
+  if (hipMalloc(&b.bw, (size_t)nb * kCap) != hipSuccess || hipMalloc(&b.bh, sizeof(uint32_t) * nr) != hipSuccess ||
+      hipMalloc(&b.bx, sizeof(uint32_t) * 2 * (size_t)(items > 0 ? items : 1)) != hipSuccess ||
+      hipMalloc(&b.bt, sizeof(uint32_t) * 256) != hipSuccess) {
+    (void)hipGetLastError();
+    free_byte(b);
+    release();
+    return 0;
+  }
+  rc = (int)hipMemsetAsync(b.bw, 0, (size_t)nb * kCap, st);
+  if (!rc) rc = (int)hipMemcpyAsync(b.bt, tbl.data(), sizeof(uint32_t) * 256, hipMemcpyHostToDevice, st);
+  if (!rc) rc = (int)hipMemcpyAsync(off, o.data(), sizeof(uint32_t) * nr, hipMemcpyHostToDevice, st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_byte_build, dim3(nb), dim3(kBlock), 0, st, pt.d, k.pk, k.base, k.stride, skey, scode, nk,
+                       cnt, off, b.bw, b.bh, reinterpret_cast<u32x2 *>(b.bx));
+    rc = (int)hipGetLastError();
+  }
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  release();
+  if (rc) { free_byte(b); return rc; }
+  dp.bw = b.bw;
+  dp.bh = b.bh;
+  dp.bx = b.bx;
+  dp.bt = b.bt;
+  dp.bItems = items;
+  dp.bFallback = fb;
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -4228,6 +4723,10 @@ static int64_t plan_bytes(const Plan *p) {
       add(d.f.pk, (int64_t)d.f.stride * d.fallback * 4);
       add(d.f.base, (int64_t)d.fallback * (int64_t)sizeof(int4));
     }
+    add(d.bw, (int64_t)x.nblocks * kCap);
+    add(d.bh, (int64_t)x.nblocks * kDRanges * 4);
+    add(d.bx, d.bItems * 8);
+    add(d.bt, 256 * 4);
   };
   delta(p->dpack, p->partT[0]);
   if (p->bundle.part.d) {
@@ -4235,6 +4734,7 @@ static int64_t plan_bytes(const Plan *p) {
     delta(p->bundle.dp, p->bundle.part);
   }
   for (const RowMap *m : {&p->rowMapT, &p->rowMapN}) {
+    add(m->fbRows, (int64_t)m->fallback * kRowChunk * 4);
     add(m->codes, (int64_t)m->chunks * kBlock * 4);
     add(m->x, (int64_t)m->chunks * kDRanges * m->E * 4);
     add(m->table, 64);
@@ -4447,6 +4947,7 @@ int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, i
     }
   }
   if (!rc && cutTOnEntries) rc = build_delta_pack(p, p->partT[0], p->packT[0], p->dpack, scratch, st);
+  if (!rc && cutTOnEntries) rc = build_byte_pack(p->partT[0], p->packT[0], p->dpack, st);
   if (!rc && cutTOnEntries) rc = tile_schedule(p, st);
   if (!rc && cutTOnEntries) rc = build_bundle(p, scratch, st);
   // a K = 1 Unsigned plan whose K = 1 kernels all read the delta pack
@@ -4460,6 +4961,18 @@ int ogs_hip_plan_create_k(ogs_hip_plan_t *plan, int32_t Ncols, int32_t NrowsN, i
       for (Pack *a : {&p->packNT, &p->packNN, &p->packS})
         if (a->pk == gone) *a = Pack();
       free_pack(p->packT[0]);
+    }
+  }
+  // ... and of its column -> row map only the rows of the row-delta map's
+  // fallback chunks (Scatter at K = 1 reads the codes; other field counts
+  // run from the CSR)
+  if (!rc && k1 && p->aliased && p->rowMapT.codes && p->colRowT) {
+    bool done = false;
+    rc = compact_rows(p->rowMapT, p->colRowT, p->Ncols, scratch, st, &done);
+    if (!rc && done) {
+      (void)hipFree(p->colRowT);
+      p->colRowT = p->colRowN = nullptr;
+      p->rowMapN = p->rowMapT;
     }
   }
   (void)hipFree(scratch);
@@ -4494,8 +5007,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->plainK1 = p->packT[0].plain;
   info->Ncols = p->Ncols;
   info->tileD = p->nparts ? p->partT[0].tileD : 0;
-  info->colsT = p->colRowT != nullptr;
-  info->colsN = p->colRowN != nullptr;
+  info->colsT = p->colRowT != nullptr || p->rowMapT.codes != nullptr;
+  info->colsN = p->colRowN != nullptr || p->rowMapN.codes != nullptr;
   info->deltaE = p->dpack.w ? p->dpack.E : 0;
   info->deltaFallback = p->dpack.w ? p->dpack.fallback : 0;
   info->ownerMap = p->colOwner != nullptr;
@@ -4509,6 +5022,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->bundleFallback = u.part.d ? u.dp.fallback : 0;
   info->rowMapE = p->rowMapT.codes ? p->rowMapT.E : 0;
   info->rowMapFallback = p->rowMapT.codes ? p->rowMapT.fallback : 0;
+  info->byteItems = p->dpack.bw ? (int32_t)p->dpack.bItems : 0;
+  info->byteFallback = p->dpack.bw ? p->dpack.bFallback : 0;
   info->bytes = plan_bytes(p);
   return OGS_SUCCESS;
 }

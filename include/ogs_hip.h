@@ -140,7 +140,7 @@ int ogs_hip_gather(ogs_hip_plan_t plan, int type, int op, int trans, int32_t K,
  * NoTrans/Sym: rows [0,NrowsT) over the T lists.  Over a list of at least
  * Ncols/2 entries (colsT / colsN of ogs_hip_plan_info) it runs by columns
  * when a column's K values take at most 16 bytes or K is not 2, 3, 4, 8 or
- * 16, else by rows.  A column held by several rows (never so for Setup-built
+ * 16, else by rows (a plan made for K=1 only: by columns at K=1).  A column held by several rows (never so for Setup-built
  * lists; the reference's row-parallel loops leave any one of their values)
  * gets the largest such row's value by columns, any one of them by rows. */
 int ogs_hip_scatter(ogs_hip_plan_t plan, int type, int trans, int32_t K,
@@ -158,7 +158,9 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * block id order).  Plan creation picks a tiled order, per kernel, only where
  * it measures faster on scratch values.  colsT / colsN: 1 where the plan
  * holds each column's row for that list (a list with at least Ncols/2
- * entries), so that Scatter can run by columns (see ogs_hip_scatter).
+ * entries), so that Scatter can run by columns (see ogs_hip_scatter); a
+ * plan made for K=1 only keeps them as row-delta codes (rowMapE > 0) and
+ * the rows of the codes' fallback chunks.
  * deltaE: exception words per 512-entry range of the K=1 GatherScatter's
  * 16-bit delta pack (0 = none; the 32-bit pack serves), deltaFallback: its
  * blocks that run from the 32-bit pack.  ownerMap: 1 where the plan holds
@@ -175,7 +177,10 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * rowMapE, rowMapFallback: exception words per 512-column range of the T
  * list's column -> row map in 4-bit row-delta codes, which Scatter at K=1
  * by columns reads (0: none, the int32 map serves), and its 2,048-column
- * chunks that read the int32 map.  bytes:
+ * chunks that read the int32 map.  byteItems, byteFallback: exception items
+ * of the K=1 GatherScatter's one-byte codes over the delta pack's blocks
+ * (both 0 and deltaE > 0: the 16-bit words serve), and their blocks that run
+ * from the 16-bit words.  bytes:
  * device memory the plan owns (partitions, packs, column maps, block
  * orders), beside the caller's CSR. */
 typedef struct {
@@ -183,7 +188,7 @@ typedef struct {
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
   int32_t colsT, colsN, deltaE, deltaFallback, ownerMap, nMask;
   int32_t bundleRy, bundleRz, bundleTy, bundleTz, bundleTileD, bundleFallback;
-  int32_t rowMapE, rowMapFallback;
+  int32_t rowMapE, rowMapFallback, byteItems, byteFallback;
   int64_t bytes;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
