@@ -1310,32 +1310,55 @@ __device__ __forceinline__ uint32_t head_bits8(uint2 x) {
   return lo | (hi << 4);
 }
 
-// Row reduction of a block staged at its CSR positions (not padded) whose
-// rows have at most 8 members.  Thread t owns positions [8t, 8t + 8); every
-// row touching them lies within [8t - 8, 8t + 16), so the thread reads its
-// own values and head bytes and its two neighbours' (before a barrier), and
-// folds in CSR order from the operator's identity: the row open at 8t
-// (begun among the previous thread's positions), its own rows, and its last
-// row on into the next thread's positions up to their first head.  Each own
+// Stage layouts of a delta-coded block, all in kCap * (sizeof(L) + 1)
+// bytes.  kPlain: the values at their CSR positions, then one head byte per
+// position (rows folded at their heads, vec_reduce); kRows: the values
+// padded (lds_pad), rows folded one per lane (delta_reduce); kSeg: groups of
+// 8 positions, each group's 8 values followed by their 8 head bytes
+// (seg8_reduce, for rows of at most 8 members: a thread reads a group of
+// 8 * sizeof(L) + 8 bytes, which puts 16 consecutive threads' groups on
+// distinct banks).
+enum { kPlain = 0, kRows = 1, kSeg = 2 };
+
+template <typename L>
+__host__ __device__ constexpr int seg_group() { return 8 * (int)sizeof(L) + 8; }
+template <typename L>
+__device__ __forceinline__ L &seg_val(uint8_t *s, int ps) {
+  return *reinterpret_cast<L *>(s + (ps >> 3) * seg_group<L>() + (ps & 7) * (int)sizeof(L));
+}
+template <typename L>
+__device__ __forceinline__ uint8_t &seg_head(uint8_t *s, int ps) {
+  return s[(ps >> 3) * seg_group<L>() + 8 * (int)sizeof(L) + (ps & 7)];
+}
+
+// Row reduction of a block staged in the kSeg layout whose rows have at most
+// 8 members.  Thread t owns positions [8t, 8t + 8) (group t); every row
+// touching them lies within groups t - 1 .. t + 1, so the thread reads those
+// three groups (before a barrier) and folds in CSR order from the operator's
+// identity: the row open at 8t (begun in group t - 1), its own rows, and its
+// last row on into group t + 1 up to that group's first head.  Each own
 // position then takes its row's result.  Positions from n on count as heads.
 // Ends with a barrier.
 template <typename T, int OP>
-__device__ __forceinline__ void seg8_reduce(int n, T *__restrict__ val, const uint8_t *__restrict__ head) {
+__device__ __forceinline__ void seg8_reduce(int n, uint8_t *__restrict__ s) {
   using R = LRed<T, OP>;
   using A = typename R::A;
-  const int P = 8 * (int)threadIdx.x;
-  const int pp = P >= 8 ? P - 8 : P, pn = P + 8 < kCap ? P + 8 : P;
-  const uint2 *h2 = reinterpret_cast<const uint2 *>(head);
-  uint32_t bits = head_bits8(h2[pp >> 3]) | (head_bits8(h2[P >> 3]) << 8) | (head_bits8(h2[pn >> 3]) << 16);
-  if (P < 8) bits |= 0xffu;  // no previous positions
+  constexpr int G = seg_group<T>(), HV = 8 * (int)sizeof(T);
+  const int t = (int)threadIdx.x, P = 8 * t;
+  const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
+  const uint8_t *sp = s + gp * G, *so = s + t * G, *sn = s + gn * G;
+  uint32_t bits = head_bits8(*reinterpret_cast<const uint2 *>(sp + HV)) |
+                  (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
+                  (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
+  if (t == 0) bits |= 0xffu;     // no previous group
   const int live = n - (P - 8);  // bit k is position P - 8 + k: from bit `live` on, past the block's entries
   if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
   // three phases, each reading 8 values: held apart (sched_barrier) so that
   // no more than 16 are in registers at once
   T x[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) x[k] = val[pp + k];
-  // the row open at P: from the previous positions' last head
+  for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(sp)[k];
+  // the row open at P: from group t - 1's last head
   const int lh = 31 - __builtin_clz(bits & 0xffu);
   A c = R::init();
 #pragma unroll
@@ -1346,7 +1369,7 @@ __device__ __forceinline__ void seg8_reduce(int n, T *__restrict__ val, const ui
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) x[k] = val[P + k];
+  for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(so)[k];
   A part[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1356,8 +1379,8 @@ __device__ __forceinline__ void seg8_reduce(int n, T *__restrict__ val, const ui
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) x[k] = val[pn + k];
-  // the last own row on to the next positions' first head
+  for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(sn)[k];
+  // the last own row on to group t + 1's first head
   const int fh = __builtin_ctz(bits >> 16);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1374,20 +1397,24 @@ __device__ __forceinline__ void seg8_reduce(int n, T *__restrict__ val, const ui
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 8; ++k) val[P + k] = res[k];
+  for (int k = 0; k < 8; ++k) reinterpret_cast<T *>(s + t * G)[k] = res[k];
   __syncthreads();
 }
 
 // The rest of a delta-coded block once a wave range's columns are known:
 // slot j of the lane holds entry 512 w + 64 j + lane, its CSR position and
 // head bit in half j & 1 of w2[j >> 1] (the 16-bit word's low 12 bits) and
-// its column in col[j].  Loads the values, stages them at their positions,
-// reduces each row in CSR order and stores the results to every member.
-template <typename T, int OP, bool ROWS>
+// its column in col[j].  Loads the values, stages them (layout MODE; val is
+// the stage, head its head bytes in kPlain and kRows), reduces each row in
+// CSR order and stores the results to every member.
+template <typename T, int OP, int MODE>
 __device__ __forceinline__ void delta_stage_reduce(const uint32_t (&w2)[4], const uint32_t (&col)[8], int n,
                                                    T *__restrict__ q, typename Lds<T>::L *__restrict__ val,
                                                    uint8_t *__restrict__ head, uint32_t *__restrict__ hm,
-                                                   int32_t *__restrict__ pc, bool seg = false) {
+                                                   int32_t *__restrict__ pc) {
+  using L = typename Lds<T>::L;
+  static_assert(MODE != kSeg || kSeg8<T>, "kSeg stages the reference's types");
+  uint8_t *sg = reinterpret_cast<uint8_t *>(val);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   T x[8];
 #pragma unroll
@@ -1398,35 +1425,50 @@ __device__ __forceinline__ void delta_stage_reduce(const uint32_t (&w2)[4], cons
     if (i0 + j * 64 < n) {
       const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
       const int ps = (int)(wj & kPackPosMask);
-      val[ROWS ? lds_pad(ps) : ps] = Lds<T>::in(x[j]);
-      head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
+      const uint8_t hb = (uint8_t)((wj >> kPackHeadBit) & 1u);
+      if constexpr (MODE == kSeg) {
+        seg_val<L>(sg, ps) = Lds<T>::in(x[j]);
+        seg_head<L>(sg, ps) = hb;
+      } else {
+        val[MODE == kRows ? lds_pad(ps) : ps] = Lds<T>::in(x[j]);
+        head[ps] = hb;
+      }
     }
   }
   __syncthreads();
-  if constexpr (ROWS) {
+  if constexpr (MODE == kRows) {
     if (n > 0) delta_reduce<T, OP>(n, val, head, hm, pc);
-  } else if constexpr (kSeg8<T>) {
-    if (seg) seg8_reduce<T, OP>(n, val, head);
-    else vec_reduce<T, OP, 1>(n, val, head, hm);
+  } else if constexpr (MODE == kSeg) {
+    seg8_reduce<T, OP>(n, sg);
   } else {
     vec_reduce<T, OP, 1>(n, val, head, hm);
   }
-  typename Lds<T>::L y[8];  // every position is in the stage: read all, then store
+  // positions again from w2 (the opaque copy keeps the compiler from holding
+  // the staging addresses across the reduction)
+  uint32_t w3[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    w3[k] = w2[k];
+    asm volatile("" : "+v"(w3[k]));
+  }
+  L y[8];  // every position is in the stage: read all, then store
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int ps = (int)((w2[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
-    y[j] = val[ROWS ? lds_pad(ps) : ps];
+    const int ps = (int)((w3[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
+    if constexpr (MODE == kSeg) y[j] = seg_val<L>(sg, ps);
+    else y[j] = val[MODE == kRows ? lds_pad(ps) : ps];
   }
 #pragma unroll
   for (int j = 0; j < 8; ++j)
     if (i0 + j * 64 < n) q[col[j]] = Lds<T>::out(y[j]);
 }
 
-// ROWS: the stage padded (lds_pad) and rows folded by delta_reduce, for
-// lists of at least two entries per row (short blocks of rows; 96^3 p=3
-// 227 -> 219 us); else staged at the plain positions and reduced at the heads
-// of the entries (48^3 p=7: the padded form measured 207 -> 230 us).
-template <typename T, int OP, bool ROWS>
+// One block of the K = 1 GatherScatter from the 16-bit delta words, staged
+// in layout MODE (val: kCap values, head4: kCap head bytes; kRows: val
+// padded).  kRows serves lists of at least two entries per row (short blocks
+// of rows; 96^3 p=3 227 -> 219 us), kSeg rows of at most 8 members, kPlain
+// the rest (48^3 p=7: the kRows form measured 207 -> 230 us).
+template <typename T, int OP, int MODE>
 __device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict__ part,
                                                    const int32_t *__restrict__ starts,
                                                    const int32_t *__restrict__ ids, const uint32_t *__restrict__ pk,
@@ -1434,8 +1476,7 @@ __device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict
                                                    const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx,
                                                    int E, uint4 tb, int ncols, T *__restrict__ q,
                                                    typename Lds<T>::L *__restrict__ val, uint32_t *__restrict__ head4,
-                                                   uint32_t *__restrict__ hm, int32_t *__restrict__ pc,
-                                                   bool seg = false) {
+                                                   uint32_t *__restrict__ hm, int32_t *__restrict__ pc) {
   using V = VecShape<T, 1>;
   static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
@@ -1445,7 +1486,7 @@ __device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict
   const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + wv) * E + lane) : 0u;
   if ((uint32_t)uniform((int)ex) == kDFallback) {
     // this block runs from its 32-bit words (every range carries the mark,
-    // and the words' slot as exception word 1)
+    // and the words' slot as exception word 1), staged plain
     const int slot = __builtin_amdgcn_readlane((int)ex, 1);
     const VecSrc src = {part, ids, pk, base, q, 0, stride};
     VecStage<V::SL> s;
@@ -1461,27 +1502,29 @@ __device__ __forceinline__ void gs_sym_delta_block(int b, const int4 *__restrict
   const uint32_t w2[4] = {wd.x, wd.y, wd.z, wd.w};
   uint32_t col[V::SL];
   delta_columns(w2, ex, tb, ncols, col);
-  delta_stage_reduce<T, OP, ROWS>(w2, col, n, q, val, head, hm, pc, seg);
+  delta_stage_reduce<T, OP, MODE>(w2, col, n, q, val, head, hm, pc);
 }
 
 #define OGS_DELTA_ARGS                                                                                         \
   const int4 *__restrict__ part, const int32_t *__restrict__ starts, const int32_t *__restrict__ ids,          \
       const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride, const u32x4 *__restrict__ dw, \
-      const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order, \
-      int seg
+      const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order
 
-// seg: rows of at most 8 members (seg8_reduce)
-template <typename T, int OP>
+// kPlain or kSeg: kCap values and kCap head bytes in one stage (also the
+// fallback path's).  kSeg held to 64 VGPRs (8 waves per SIMD; unbounded it
+// took 69).
+template <typename T, int OP, int MODE>
 __global__ __launch_bounds__(kBlock) void k_gs_sym_delta(OGS_DELTA_ARGS) {
-  __shared__ typename Lds<T>::L val[kCap];  // K = 1: at most kCap values, also on the fallback path
-  __shared__ uint32_t head4[kCap / 4];
+  using L = typename Lds<T>::L;
+  __shared__ __align__(16) uint8_t stage[kCap * (sizeof(L) + 1)];
   __shared__ uint32_t hm[kCap / 32 + 1];
   const int xb = xcd_block(blockIdx.x, gridDim.x);
-  gs_sym_delta_block<T, OP, false>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols,
-                                   q, val, head4, hm, nullptr, seg != 0);
+  gs_sym_delta_block<T, OP, MODE>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols, q,
+                                  reinterpret_cast<L *>(stage), reinterpret_cast<uint32_t *>(stage + kCap * sizeof(L)),
+                                  hm, nullptr);
 }
 
-// The ROWS form (lists of >= 2 entries per row), held to 64 VGPRs (8 waves
+// The kRows form (lists of >= 2 entries per row), held to 64 VGPRs (8 waves
 // per SIMD; unbounded it took 69).
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta_rows(OGS_DELTA_ARGS) {
@@ -1490,8 +1533,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_gs_sym_delta_rows(OGS_DELTA_ARGS)
   __shared__ uint32_t hm[kCap / 32 + 1];
   __shared__ int32_t pc[kCap / 32];
   const int xb = xcd_block(blockIdx.x, gridDim.x);
-  gs_sym_delta_block<T, OP, true>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols,
-                                  q, val, head4, hm, pc);
+  gs_sym_delta_block<T, OP, kRows>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols,
+                                   q, val, head4, hm, pc);
 }
 #undef OGS_DELTA_ARGS
 
@@ -1553,23 +1596,26 @@ __device__ __forceinline__ void byte_columns(u32x2 cw, u32x2 ex, const uint32_t 
   }
 }
 
-template <typename T, int OP>
+// MODE kPlain or kSeg, as k_gs_sym_delta.
+template <typename T, int OP, int MODE>
 __global__ __launch_bounds__(kBlock) void k_gs_sym_byte(
     const int4 *__restrict__ part, const int32_t *__restrict__ starts, const int32_t *__restrict__ ids,
     const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride, const u32x4 *__restrict__ dw,
     const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q, const int *__restrict__ order,
-    int seg, const u32x2 *__restrict__ bw, const uint4 *__restrict__ bh, const u32x2 *__restrict__ bx,
+    const u32x2 *__restrict__ bw, const uint4 *__restrict__ bh, const u32x2 *__restrict__ bx,
     const uint32_t *__restrict__ bt) {
-  __shared__ typename Lds<T>::L val[kCap];
-  __shared__ uint32_t head4[kCap / 4];
+  using L = typename Lds<T>::L;
+  __shared__ __align__(16) uint8_t stage[kCap * (sizeof(L) + 1)];
   __shared__ uint32_t hm[kCap / 32 + 1];
   __shared__ uint32_t tab[256];
+  L *val = reinterpret_cast<L *>(stage);
+  uint32_t *head4 = reinterpret_cast<uint32_t *>(stage + kCap * sizeof(L));
   const int xb = xcd_block(blockIdx.x, gridDim.x);
   const int b = order ? order[xb] : xb;
   const uint4 h = bh[b];
   if (h.x == kDFallback) {
-    gs_sym_delta_block<T, OP, false>(b, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols, q, val, head4, hm,
-                                     nullptr, seg != 0);
+    gs_sym_delta_block<T, OP, MODE>(b, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols, q, val, head4, hm,
+                                    nullptr);
     return;
   }
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1583,7 +1629,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte(
   __syncthreads();
   uint32_t col[8], w2[4];
   byte_columns(cw, ex, tab, ncols, col, w2);
-  delta_stage_reduce<T, OP, false>(w2, col, n, q, val, reinterpret_cast<uint8_t *>(head4), hm, nullptr, seg != 0);
+  delta_stage_reduce<T, OP, MODE>(w2, col, n, q, val, reinterpret_cast<uint8_t *>(head4), hm, nullptr);
 }
 
 // GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
@@ -3340,17 +3386,29 @@ static void launch_delta_on(const Partition &pt, const DeltaPack &d, const int32
                             int ncols, T *v, const int *order, hipStream_t st) {
   const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
   const u32x4 *dw = reinterpret_cast<const u32x4 *>(d.w);
-  const int seg = d.seg8 ? 1 : 0;
-  if (d.bw)
-    hipLaunchKernelGGL((k_gs_sym_byte<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk, d.f.base,
-                       d.f.stride, dw, d.x, d.E, tb, ncols, v, order, seg, reinterpret_cast<const u32x2 *>(d.bw),
-                       reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
+  const dim3 grid(pt.nblocks), block(kBlock);
+  auto byte = [&](auto mode) {
+    hipLaunchKernelGGL((k_gs_sym_byte<T, OP, decltype(mode)::value>), grid, block, 0, st, pt.d, rs, ci, d.f.pk,
+                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order,
+                       reinterpret_cast<const u32x2 *>(d.bw), reinterpret_cast<const uint4 *>(d.bh),
+                       reinterpret_cast<const u32x2 *>(d.bx), d.bt);
+  };
+  if constexpr (kSeg8<T>) {
+    if (d.seg8) {  // rows of at most 8 members
+      if (d.bw) byte(std::integral_constant<int, kSeg>{});
+      else
+        hipLaunchKernelGGL((k_gs_sym_delta<T, OP, kSeg>), grid, block, 0, st, pt.d, rs, ci, d.f.pk, d.f.base,
+                           d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
+      return;
+    }
+  }
+  if (d.bw && !d.rows) byte(std::integral_constant<int, kPlain>{});
   else if (d.rows)
-    hipLaunchKernelGGL((k_gs_sym_delta_rows<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
-                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order, 0);
+    hipLaunchKernelGGL((k_gs_sym_delta_rows<T, OP>), grid, block, 0, st, pt.d, rs, ci, d.f.pk, d.f.base, d.f.stride,
+                       dw, d.x, d.E, tb, ncols, v, order);
   else
-    hipLaunchKernelGGL((k_gs_sym_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, rs, ci, d.f.pk,
-                       d.f.base, d.f.stride, dw, d.x, d.E, tb, ncols, v, order, seg);
+    hipLaunchKernelGGL((k_gs_sym_delta<T, OP, kPlain>), grid, block, 0, st, pt.d, rs, ci, d.f.pk, d.f.base,
+                       d.f.stride, dw, d.x, d.E, tb, ncols, v, order);
 }
 
 // ... over the plan's bundled blocks where it has them, else partition 0.
@@ -4261,12 +4319,13 @@ static int build_delta_pack(const Plan *p, const Partition &pt, const Pack &k, D
 // the table from the triples of up to 512 sample blocks spread over the
 // partition (the 255 commonest), each range's exception items counted, their
 // offsets summed on the host, then the codes.  Built for blocks that fold at
-// the heads of their entries (fewer than 2 entries per row); none when the
-// device has no room or more than a quarter of the blocks would fall back.
+// the heads of their entries or in groups of 8 (not for the kRows form);
+// none when the device has no room or more than a quarter of the blocks
+// would fall back.
 constexpr int kByteSample = 512;
 
 static int build_byte_pack(const Partition &pt, const Pack &k, DeltaPack &dp, hipStream_t st) {
-  if (!dp.w || dp.rows || !k.pk || pt.nblocks == 0) return 0;
+  if (!dp.w || (dp.rows && !dp.seg8) || !k.pk || pt.nblocks == 0) return 0;
   const int nb = pt.nblocks;
   const int every = nb > kByteSample ? nb / kByteSample : 1;
   const int ns = (nb - 1) / every + 1;
@@ -4634,6 +4693,7 @@ static int build_bundle(Plan *p, int32_t *scratch, hipStream_t st) {
     bool ok = !rc && x.k.pk && x.k.plain == 0;
     if (ok) rc = build_delta_pack(p, u.part, x.k, u.dp, scratch, st);
     ok = ok && !rc && u.dp.w;
+    if (ok) rc = build_byte_pack(u.part, x.k, u.dp, st);
     int D = 0;
     if (ok) rc = layer_distance(p, u.part, ciP, st, &D);
     if (rsP) (void)hipFree(rsP);

@@ -1,8 +1,10 @@
 """GPU parity of the K = 1 one-list GatherScatter over the plan's 16-bit
-delta pack (k_gs_sym_delta): bit-exact against the oracle on meshes large
-enough for the pack (partition 0 fills the device), for every type and op,
-Unsigned and Signed (Sym), with fallback blocks (columns without short runs)
-mixed in, and with the pack dropped when most blocks would fall back.
+delta pack (k_gs_sym_delta) and its one-byte codes (k_gs_sym_byte):
+bit-exact against the oracle on meshes large enough for the pack
+(partition 0 fills the device), for every type and op, Unsigned and Signed
+(Sym), with fallback blocks (columns without short runs) mixed in, with the
+pack dropped when most blocks would fall back, and with rows of more than 8
+members (which the 8-position group fold does not take).
 """
 import numpy as np
 import pytest
@@ -86,6 +88,40 @@ def test_delta_gather_scatter(torch, D, box, ty, op):
     g.GatherScatter(d, 1, op, D.Sym)
     torch.cuda.synchronize()
     assert same(d.cpu().numpy(), want), (box, np.dtype(dt).name, op)
+
+
+def test_byte_codes_built(D):
+    """48^3-like p=7 numbering (16^3 elements here): the one-byte codes exist
+    over the delta pack's blocks, most blocks encoded (48^3: 116 of 27,743
+    fall back)."""
+    g, _, _ = unsigned_mesh(D, (16, 16, 16, 7))
+    info = g.plan.info()
+    assert info["maxRowT"] == 8 and info["byteItems"] > 0, info
+    assert info["byteFallback"] <= info["nblocksK1"] // 16, info
+
+
+@pytest.mark.parametrize("ty", list(TYPES))
+@pytest.mark.parametrize("op", [0, 2])
+def test_delta_rows_over_8(torch, D, ty, op):
+    """Ids of a 16^3 p=7 mesh with every third pair of global ids merged
+    (rows of up to 12 members): the kernels fold at the heads of the
+    entries instead of in groups of 8; bit-exact."""
+    ids = O.make_ids(16, 16, 16, 7)
+    g0 = ids - 1
+    merged = np.where((g0 // 2) % 3 == 0, (g0 // 2) * 2, g0)
+    g, _ = setup(D, merged + 1)
+    info = g.plan.info()
+    assert info["maxRowT"] > 8 and info["deltaE"] > 0, info
+    c = g.csr()
+    ref = O.OracleCsr(c["rowStartsT"], c["colIdsT"], Ncols=len(ids))
+    dt = TYPES[ty]
+    rng = np.random.default_rng(1000 + 10 * ty + op)
+    v = values(rng, dt, len(ids))
+    want = ref.gather_scatter(v.copy(), 1, op, O.SYM)
+    d = torch.from_numpy(v.copy()).cuda()
+    g.GatherScatter(d, 1, op, D.Sym)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), want), (np.dtype(dt).name, op)
 
 
 @pytest.mark.parametrize("ty,op", [(O.FLOAT_C, 0), (O.BF16, 0), (O.BF16, 2), (O.FLOAT_C, 3)])
