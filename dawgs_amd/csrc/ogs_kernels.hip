@@ -1331,23 +1331,23 @@ __device__ __forceinline__ uint8_t &seg_head(uint8_t *s, int ps) {
   return s[(ps >> 3) * seg_group<L>() + 8 * (int)sizeof(L) + (ps & 7)];
 }
 
-// Row reduction of a block staged in the kSeg layout whose rows have at most
-// 8 members.  Thread t owns positions [8t, 8t + 8) (group t); every row
+// Row results of a block staged in the kSeg layout whose rows have at most 8
+// members.  Thread t owns positions [8t, 8t + 8) (group t); every row
 // touching them lies within groups t - 1 .. t + 1, so the thread reads those
-// three groups (before a barrier) and folds in CSR order from the operator's
-// identity: the row open at 8t (begun in group t - 1), its own rows, and its
-// last row on into group t + 1 up to that group's first head.  Each own
-// position then takes its row's result.  Positions from n on count as heads.
-// Ends with a barrier.
+// three groups and folds in CSR order from the operator's identity: the row
+// open at 8t (begun in group t - 1), its own rows, and its last row on into
+// group t + 1 up to that group's first head.  res[k]: the result of the row
+// holding position 8t + k; bits: head bits of positions 8t - 8 .. 8t + 15
+// (bit k: position 8t - 8 + k), positions from n on counted as heads.
 template <typename T, int OP>
-__device__ __forceinline__ void seg8_reduce(int n, uint8_t *__restrict__ s) {
+__device__ __forceinline__ void seg8_fold(int n, const uint8_t *__restrict__ s, T (&res)[8], uint32_t &bits) {
   using R = LRed<T, OP>;
   using A = typename R::A;
   constexpr int G = seg_group<T>(), HV = 8 * (int)sizeof(T);
   const int t = (int)threadIdx.x, P = 8 * t;
   const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
   const uint8_t *sp = s + gp * G, *so = s + t * G, *sn = s + gn * G;
-  uint32_t bits = head_bits8(*reinterpret_cast<const uint2 *>(sp + HV)) |
+  bits = head_bits8(*reinterpret_cast<const uint2 *>(sp + HV)) |
                   (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
                   (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
   if (t == 0) bits |= 0xffu;     // no previous group
@@ -1388,17 +1388,56 @@ __device__ __forceinline__ void seg8_reduce(int n, uint8_t *__restrict__ s) {
     R::apply(y, x[k]);
     c = k < fh ? y : c;
   }
-  T res[8];
   res[7] = R::fin(c);
 #pragma unroll
   for (int k = 7; k > 0; --k) {
     if ((bits >> (8 + k)) & 1u) c = part[k - 1];
     res[k - 1] = R::fin(c);
   }
+}
+
+// seg8_fold, each own position then taking its row's result (written after
+// a barrier, since neighbours read the group).  Ends with a barrier.
+template <typename T, int OP>
+__device__ __forceinline__ void seg8_reduce(int n, uint8_t *__restrict__ s) {
+  T res[8];
+  uint32_t bits;
+  seg8_fold<T, OP>(n, s, res, bits);
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 8; ++k) reinterpret_cast<T *>(s + t * G)[k] = res[k];
+  for (int k = 0; k < 8; ++k) reinterpret_cast<T *>(s + threadIdx.x * seg_group<T>())[k] = res[k];
   __syncthreads();
+}
+
+// Gather from a kSeg stage: seg8_fold, then the thread holding a row's head
+// puts the row's result at the row's index in the block (the heads before
+// it: per thread, summed over the wave by a scan and over the waves before
+// through wsum) in the stage, and the block stores its rows' results in
+// order, whole lines (stored from the heads directly, rows of neighbouring
+// lanes lie ~5 apart at p=7 and each store instruction writes parts of ~20
+// lines: 48^3 p=7 164 -> 205 us).
+template <typename T, int OP>
+__device__ __forceinline__ void seg8_gather(int n, int r0, uint8_t *__restrict__ s, uint32_t *__restrict__ wsum,
+                                            T *__restrict__ gq) {
+  T res[8];
+  uint32_t bits;
+  seg8_fold<T, OP>(n, s, res, bits);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int own = n - 8 * (int)threadIdx.x;  // own positions below n
+  const uint32_t hh = ((bits >> 8) & 0xffu) & (own >= 8 ? 0xffu : own > 0 ? (1u << own) - 1u : 0u);
+  const uint32_t hc = (uint32_t)__popc(hh);
+  const uint32_t inc = wave_scan_incl(hc);
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();  // (also: every thread's reads of the stage are done)
+  uint32_t row = inc - hc;
+  for (int w = 0; w < wv; ++w) row += wsum[w];
+  const int rows = (int)(wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+  T *rb = reinterpret_cast<T *>(s);  // rows <= n <= kCap results
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if ((hh >> k) & 1u) rb[row + __popc(hh & ((1u << k) - 1u))] = res[k];
+  __syncthreads();
+  for (int i = threadIdx.x; i < rows; i += kBlock) gq[(size_t)r0 + i] = rb[i];
 }
 
 // The rest of a delta-coded block once a wave range's columns are known:
@@ -1798,30 +1837,60 @@ __global__ __launch_bounds__(kBlock) void k_gather_vec(const int4 *__restrict__ 
   gather_vec_block<T, OP, KT>(src, b, b, starts, gq, val, reinterpret_cast<uint8_t *>(head4), hm, pc);
 }
 
-// Gather K = 1 over partition 0 from the delta pack (see k_gs_sym_delta):
-// columns decoded per wave, values staged at their CSR positions, rows
-// reduced and stored as in k_gather_vec; fallback blocks run k_gather_vec's
-// body from the 32-bit pack.
-template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_gather_delta(const int4 *__restrict__ part,
-                                                         const int32_t *__restrict__ starts,
-                                                         const int32_t *__restrict__ ids,
-                                                         const uint32_t *__restrict__ pk,
-                                                         const int4 *__restrict__ base, int stride,
-                                                         const u32x4 *__restrict__ dw,
-                                                         const uint32_t *__restrict__ dx, int E, uint4 tb,
-                                                         int ncols, const T *__restrict__ q, T *__restrict__ gq,
-                                                         const int *__restrict__ order) {
+// The values of a delta-coded Gather block once a wave range's columns are
+// known (as delta_stage_reduce), staged in layout MODE (kPlain: val and
+// head; kSeg: the group stage at val) and reduced to gq[r0 + row].
+template <typename T, int OP, int MODE>
+__device__ __forceinline__ void gather_stage(const uint32_t (&w2)[4], const uint32_t (&col)[8], int n, int r0,
+                                             const T *__restrict__ q, T *__restrict__ gq,
+                                             typename Lds<T>::L *__restrict__ val, uint8_t *__restrict__ head,
+                                             uint32_t *__restrict__ hm, int32_t *__restrict__ pc,
+                                             uint32_t *__restrict__ wsum) {
+  using L = typename Lds<T>::L;
+  static_assert(MODE != kSeg || kSeg8<T>, "kSeg stages the reference's types");
+  uint8_t *sg = reinterpret_cast<uint8_t *>(val);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  T x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = q[col[j]];
+  const int i0 = wv * kDWave + lane;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (i0 + j * 64 < n) {
+      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
+      const int ps = (int)(wj & kPackPosMask);
+      const uint8_t hb = (uint8_t)((wj >> kPackHeadBit) & 1u);
+      if constexpr (MODE == kSeg) {
+        seg_val<L>(sg, ps) = Lds<T>::in(x[j]);
+        seg_head<L>(sg, ps) = hb;
+      } else {
+        val[ps] = Lds<T>::in(x[j]);
+        head[ps] = hb;
+      }
+    }
+  }
+  __syncthreads();
+  if constexpr (MODE == kSeg) seg8_gather<T, OP>(n, r0, sg, wsum, gq);
+  else gather_rows<T, OP, 1>(n, r0, val, head, hm, pc, gq);
+}
+
+// Gather K = 1 of block b over partition 0 from the 16-bit delta words (see
+// k_gs_sym_delta): columns decoded per wave, values staged, rows reduced and
+// stored as in k_gather_vec; fallback blocks run k_gather_vec's body from
+// the 32-bit pack (staged plain: val holds kCap values, head4 kCap bytes).
+template <typename T, int OP, int MODE>
+__device__ __forceinline__ void gather_delta_block(int b, const int4 *__restrict__ part,
+                                                   const int32_t *__restrict__ starts,
+                                                   const int32_t *__restrict__ ids, const uint32_t *__restrict__ pk,
+                                                   const int4 *__restrict__ base, int stride,
+                                                   const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx,
+                                                   int E, uint4 tb, int ncols, const T *__restrict__ q,
+                                                   T *__restrict__ gq, typename Lds<T>::L *__restrict__ val,
+                                                   uint32_t *__restrict__ head4, uint32_t *__restrict__ hm,
+                                                   int32_t *__restrict__ pc, uint32_t *__restrict__ wsum) {
   using V = VecShape<T, 1>;
   static_assert(V::SL == 8 && V::C == 1 && V::CAP == kCap, "delta pack: one entry per slot, kCap per block");
-  using L = typename Lds<T>::L;
-  __shared__ L val[vec_cap<L>()];
-  __shared__ uint32_t head4[kCap / 4];
-  __shared__ uint32_t hm[kCap / 32 + 1];
-  __shared__ int32_t pc[kCap / 32];
   uint8_t *head = reinterpret_cast<uint8_t *>(head4);
-  const int xb = xcd_block(blockIdx.x, gridDim.x);
-  const int b = order ? order[xb] : xb;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int4 p0 = part[b], p1 = part[b + 1];
   const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
@@ -1837,22 +1906,66 @@ __global__ __launch_bounds__(kBlock) void k_gather_delta(const int4 *__restrict_
   const uint32_t w2[4] = {wd.x, wd.y, wd.z, wd.w};
   uint32_t col[V::SL];
   delta_columns(w2, ex, tb, ncols, col);
-  T x[V::SL];
-#pragma unroll
-  for (int j = 0; j < V::SL; ++j) x[j] = q[col[j]];
-  const int i0 = wv * kDWave + lane;
-#pragma unroll
-  for (int j = 0; j < V::SL; ++j) {
-    if (i0 + j * 64 < n) {
-      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
-      const int ps = (int)(wj & kPackPosMask);
-      val[ps] = Lds<T>::in(x[j]);
-      head[ps] = (uint8_t)((wj >> kPackHeadBit) & 1u);
-    }
-  }
-  __syncthreads();
-  gather_rows<T, OP, 1>(n, r0, val, head, hm, pc, gq);
+  gather_stage<T, OP, MODE>(w2, col, n, r0, q, gq, val, head, hm, pc, wsum);
 }
+
+#define OGS_GATHER_DELTA_ARGS                                                                                  \
+  const int4 *__restrict__ part, const int32_t *__restrict__ starts, const int32_t *__restrict__ ids,          \
+      const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride, const u32x4 *__restrict__ dw, \
+      const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, const T *__restrict__ q, T *__restrict__ gq, \
+      const int *__restrict__ order
+
+// MODE kPlain or kSeg (rows of at most 8 members).
+template <typename T, int OP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_gather_delta(OGS_GATHER_DELTA_ARGS) {
+  using L = typename Lds<T>::L;
+  __shared__ __align__(16) uint8_t stage[kCap * (sizeof(L) + 1)];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
+  __shared__ uint32_t wsum[kDRanges];
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  gather_delta_block<T, OP, MODE>(order ? order[xb] : xb, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols, q,
+                                  gq, reinterpret_cast<L *>(stage),
+                                  reinterpret_cast<uint32_t *>(stage + kCap * sizeof(L)), hm, pc, wsum);
+}
+
+// Gather K = 1 from the byte codes (see k_gs_sym_byte); blocks the codes do
+// not cover run gather_delta_block.
+template <typename T, int OP, int MODE>
+__global__ __launch_bounds__(kBlock) void k_gather_byte(OGS_GATHER_DELTA_ARGS, const u32x2 *__restrict__ bw,
+                                                        const uint4 *__restrict__ bh, const u32x2 *__restrict__ bx,
+                                                        const uint32_t *__restrict__ bt) {
+  using L = typename Lds<T>::L;
+  __shared__ __align__(16) uint8_t stage[kCap * (sizeof(L) + 1)];
+  __shared__ uint32_t hm[kCap / 32 + 1];
+  __shared__ int32_t pc[kCap / 32];
+  __shared__ uint32_t wsum[kDRanges];
+  __shared__ uint32_t tab[256];
+  L *val = reinterpret_cast<L *>(stage);
+  uint32_t *head4 = reinterpret_cast<uint32_t *>(stage + kCap * sizeof(L));
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const uint4 h = bh[b];
+  if (h.x == kDFallback) {
+    gather_delta_block<T, OP, MODE>(b, part, starts, ids, pk, base, stride, dw, dx, E, tb, ncols, q, gq, val, head4, hm,
+                                    pc, wsum);
+    return;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const u32x2 cw = __builtin_nontemporal_load(bw + (size_t)b * kBlock + tid);
+  tab[tid] = bt[tid];
+  const uint32_t hr = (uint32_t)uniform((int)(wv == 0 ? h.x : wv == 1 ? h.y : wv == 2 ? h.z : h.w));
+  const int cnt = (int)(hr & 127u);
+  u32x2 ex = {0u, 0u};
+  if (lane < cnt) ex = __builtin_nontemporal_load(bx + (size_t)(hr >> 7) + lane);
+  const int r0 = uniform(part[b].x);
+  const int n = uniform(part[b + 1].y) - uniform(part[b].y);
+  __syncthreads();
+  uint32_t col[8], w2[4];
+  byte_columns(cw, ex, tab, ncols, col, w2);
+  gather_stage<T, OP, MODE>(w2, col, n, r0, q, gq, val, reinterpret_cast<uint8_t *>(head4), hm, pc, wsum);
+}
+#undef OGS_GATHER_DELTA_ARGS
 
 // Scatter of a T list without empty rows, KT interleaved fields: the block's
 // rows (KT values each, contiguous) are staged in LDS; an item's row is the
@@ -3596,15 +3709,40 @@ static int launch_scatter_cols(int32_t ncols, int K, const int32_t *colRow, cons
   return (int)hipGetLastError();
 }
 
+// Gather K = 1 from delta pack d over partition pt (CSR rs, ci): from the
+// byte codes where the plan has them, staged in groups of 8 for rows of at
+// most 8 members.
+template <typename T, int OP>
+static void launch_gather_delta_on(const Partition &pt, const DeltaPack &d, const int32_t *rs, const int32_t *ci,
+                                   int ncols, const T *v, T *gv, const int *order, hipStream_t st) {
+  const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
+  const u32x4 *dw = reinterpret_cast<const u32x4 *>(d.w);
+  const dim3 grid(pt.nblocks), block(kBlock);
+  auto go = [&](auto mode) {
+    constexpr int M = decltype(mode)::value;
+    if (d.bw)
+      hipLaunchKernelGGL((k_gather_byte<T, OP, M>), grid, block, 0, st, pt.d, rs, ci, d.f.pk, d.f.base, d.f.stride,
+                         dw, d.x, d.E, tb, ncols, v, gv, order, reinterpret_cast<const u32x2 *>(d.bw),
+                         reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
+    else
+      hipLaunchKernelGGL((k_gather_delta<T, OP, M>), grid, block, 0, st, pt.d, rs, ci, d.f.pk, d.f.base, d.f.stride,
+                         dw, d.x, d.E, tb, ncols, v, gv, order);
+  };
+  if constexpr (kSeg8<T>) {
+    if (d.seg8) {
+      go(std::integral_constant<int, kSeg>{});
+      return;
+    }
+  }
+  go(std::integral_constant<int, kPlain>{});
+}
+
 // Gather K = 1 of the T list over partition 0 from the delta pack.
 template <typename T, int OP>
 static void launch_gather_delta_op(const Plan *p, const T *v, T *gv, hipStream_t st) {
   const Partition &pt = p->partT[0];
-  const DeltaPack &d = p->dpack;
-  hipLaunchKernelGGL((k_gather_delta<T, OP>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT, p->ciT, d.f.pk,
-                     d.f.base, d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E,
-                     make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]), p->Ncols, v, gv,
-                     (pt.tiled & kTiledGather) ? pt.order : nullptr);
+  launch_gather_delta_on<T, OP>(pt, p->dpack, p->rsT, p->ciT, p->Ncols, v, gv,
+                                (pt.tiled & kTiledGather) ? pt.order : nullptr, st);
 }
 
 template <typename T>
@@ -3916,8 +4054,6 @@ static int tile_schedule(Plan *p, hipStream_t st) {
     rc = upload(pt, DT, &oT);
     const Pack &k = p->packT[0];
     const DeltaPack &dp = p->dpack;
-    const u32x4 *dw = reinterpret_cast<const u32x4 *>(dp.w);
-    const uint4 tb = make_uint4(dp.tbl[0], dp.tbl[1], dp.tbl[2], dp.tbl[3]);
     // the kernels the K = 1 launches will run: over the delta pack when the plan has one
     if (!rc)
       rc = order_pays([&](const int *o) {
@@ -3930,8 +4066,7 @@ static int tile_schedule(Plan *p, hipStream_t st) {
     if (!rc)
       rc = order_pays([&](const int *o) {
         if (dp.w)
-          hipLaunchKernelGGL((k_gather_delta<double, OGS_ADD>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, p->rsT,
-                             p->ciT, dp.f.pk, dp.f.base, dp.f.stride, dw, dp.x, dp.E, tb, p->Ncols, vals, rows, o);
+          launch_gather_delta_on<double, OGS_ADD>(pt, dp, p->rsT, p->ciT, p->Ncols, vals, rows, o, st);
         else
           hipLaunchKernelGGL((k_gather_vec<double, OGS_ADD, 1>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d,
                              p->rsT, p->ciT, k.pk, k.base, k.stride, vals, rows, o);
