@@ -635,7 +635,7 @@ def main() -> int:
     B = algorithmic_bytes(ogs.nnzT, ogs.NrowsT, 1, 8)
     achieved = B / (ms * 1e-3) / 1e9
     # measured HBM bytes per launch (two rocprofv3 --pmc runs, scaled by a
-    # copy kernel of known size; profiles/k_gs_sym_vec_f64_add_48cubed_p7.md)
+    # copy kernel of known size; profiles/k_gs_sym_byte_f64_add_48cubed_p7.md)
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and (nx, ny, nz, p) == (48, 48, 48, 7):
         with open(TRAFFIC_FILE) as f:
