@@ -358,7 +358,14 @@ def test_bundled_blocks(torch, D, box, fields):
         g.GatherScatter(d, 1, op, D.Sym)
         torch.cuda.synchronize()
         assert same(d.cpu().numpy(), want), (box, np.dtype(dt).name, op)
-        del d, want
+        # the Gather on the same plan (partition 0, byte codes, rows of at
+        # most 8 members folded per group of 8 positions)
+        wg = ref.gather(v, 1, op, O.TRANS)
+        gq = torch.zeros(len(wg), dtype=torch.from_numpy(v[:1]).dtype, device="cuda")
+        g.plan.Gather(gq, torch.from_numpy(v).cuda(), 1, op, D.Trans)
+        torch.cuda.synchronize()
+        assert same(gq.cpu().numpy(), wg), (box, np.dtype(dt).name, op)
+        del d, want, gq, wg
 
 
 def test_bundled_blocks_signed_operator(torch, D):
