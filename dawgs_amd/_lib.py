@@ -21,7 +21,7 @@ class PlanInfo(C.Structure):
                 ("NrowsN", "NrowsT", "nnzN", "nnzT", "maxRowT", "maxRowN", "nblocksK1", "widthK1",
                  "packedK1", "plainK1", "Ncols", "tileD", "colsT", "colsN", "deltaE", "deltaFallback",
                  "ownerMap", "nMask", "bundleRy", "bundleRz", "bundleTy", "bundleTz", "bundleTileD",
-                 "bundleFallback", "rowMapE", "rowMapFallback", "byteItems", "byteFallback")] + \
+                 "bundleFallback", "rowMapE", "rowMapFallback", "byteItems", "byteFallback", "byteK3")] + \
                [("bytes", C.c_int64)]
 
 
@@ -31,7 +31,7 @@ class BatchDesc(C.Structure):
 
 class BatchInfo(C.Structure):
     _fields_ = [("items", C.c_int32), ("nodes", C.c_int32), ("narrow", C.c_int32), ("order", C.c_int32), ("fused", C.c_int32),
-                ("candidates", C.c_int32), ("tune_ms", C.c_float), ("best_us", C.c_float)]
+                ("candidates", C.c_int32), ("tune_ms", C.c_float), ("best_us", C.c_float), ("plans", C.c_int32)]
 
 
 class OgsInfo(C.Structure):

@@ -82,6 +82,7 @@ struct DeltaPack {
   int32_t E = 0;                       // exception words per 512-entry range
   uint32_t tbl[4] = {0u, 0u, 0u, 0u};  // byte c: the delta of code c
   int32_t fallback = 0;                // blocks that run from the 32-bit words
+  int32_t csr = 0;                     // of those, blocks left in CSR order (words: column ids)
   Pack f;                              // the fallback blocks' 32-bit words
   bool ownsF = false;                  // f is the compacted copy (else the partition's pack)
   bool rows = false;                   // GatherScatter folds compacted rows (>= 2 entries per row)

@@ -1671,6 +1671,189 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte(
   delta_stage_reduce<T, OP, MODE>(w2, col, n, q, val, reinterpret_cast<uint8_t *>(head4), hm, nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// The byte pack's blocks with KT interleaved fields: the one-list
+// GatherScatter Sym at K = KT on lists whose rows have at most 8 members.
+// One decode of a block's codes (the K = 1 kernel's) feeds KT contiguous
+// value loads per entry, so the index costs ~1 byte per entry instead of the
+// 4-byte pack word of the entry-major kernel, whose blocks hold only as many
+// entries as 16 KB of values (682 at K = 3 in fp64).  The stage holds the
+// block's kCap / 8 groups of 8 CSR positions, each group's KT x 8 values
+// (field-major) followed by its 8 head bytes; thread t folds group t for
+// every field (rows read from groups t - 1 .. t + 1, as seg8_fold), holds
+// the results until every thread has read the stage, and writes them back.
+// Blocks without codes run from their 16-bit words, and blocks without those
+// from their 32-bit words: the launcher runs this kernel only on packs that
+// left no block in CSR order.
+template <typename L, int KT>
+__host__ __device__ constexpr int segk_group() { return KT * 8 * (int)sizeof(L) + 8; }
+
+// Rows of the group-t fold of field f (bits as seg8_fold builds them); res[k]
+// is the result of the row holding position 8t + k.
+template <typename T, int OP, int KT>
+__device__ __forceinline__ void segk_fold(const uint8_t *__restrict__ s, int f, uint32_t bits,
+                                          typename Lds<T>::L (&res)[8]) {
+  using L = typename Lds<T>::L;
+  using R = LRed<T, OP>;
+  using A = typename R::A;
+  constexpr int G = segk_group<L, KT>();
+  const int t = (int)threadIdx.x;
+  const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
+  const L *sp = reinterpret_cast<const L *>(s + gp * G) + 8 * f;
+  const L *so = reinterpret_cast<const L *>(s + t * G) + 8 * f;
+  const L *sn = reinterpret_cast<const L *>(s + gn * G) + 8 * f;
+  L x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = sp[k];
+  // the row open at 8t: from group t - 1's last head
+  const int lh = 31 - __builtin_clz(bits & 0xffu);
+  A c = R::init();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    A y = c;
+    R::apply(y, x[k]);
+    c = k >= lh ? y : c;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = so[k];
+  A part[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if ((bits >> (8 + k)) & 1u) c = R::init();
+    R::apply(c, x[k]);
+    part[k] = c;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = sn[k];
+  // the last own row on to group t + 1's first head
+  const int fh = __builtin_ctz(bits >> 16);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    A y = c;
+    R::apply(y, x[k]);
+    c = k < fh ? y : c;
+  }
+  res[7] = R::fin(c);
+#pragma unroll
+  for (int k = 7; k > 0; --k) {
+    if ((bits >> (8 + k)) & 1u) c = part[k - 1];
+    res[k - 1] = R::fin(c);
+  }
+}
+
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
+    const int4 *__restrict__ part, const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride,
+    const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q,
+    const int *__restrict__ order, const u32x2 *__restrict__ bw, const uint4 *__restrict__ bh,
+    const u32x2 *__restrict__ bx, const uint32_t *__restrict__ bt) {
+  using L = typename Lds<T>::L;
+  constexpr int G = segk_group<L, KT>(), HV = KT * 8 * (int)sizeof(L);
+  __shared__ __align__(16) uint8_t stage[(kCap / 8) * G];
+  __shared__ uint32_t tab[256];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const int n = uniform(part[b + 1].y) - uniform(part[b].y);
+  const uint4 h = bh[b];
+  uint32_t col[8], w2[4];
+  if (uniform((int)h.x) != (int)kDFallback) {
+    const u32x2 cw = __builtin_nontemporal_load(bw + (size_t)b * kBlock + tid);
+    tab[tid] = bt[tid];
+    const uint32_t hr = (uint32_t)uniform((int)(wv == 0 ? h.x : wv == 1 ? h.y : wv == 2 ? h.z : h.w));
+    const int cnt = (int)(hr & 127u);
+    u32x2 ex = {0u, 0u};
+    if (lane < cnt) ex = __builtin_nontemporal_load(bx + (size_t)(hr >> 7) + lane);
+    __syncthreads();
+    byte_columns(cw, ex, tab, ncols, col, w2);
+  } else {
+    const u32x4 wd = __builtin_nontemporal_load(dw + (size_t)b * kBlock + tid);
+    const uint32_t ex = lane < E ? __builtin_nontemporal_load(dx + ((size_t)b * kDRanges + wv) * E + lane) : 0u;
+    if ((uint32_t)uniform((int)ex) != kDFallback) {
+      w2[0] = wd.x; w2[1] = wd.y; w2[2] = wd.z; w2[3] = wd.w;
+      delta_columns(w2, ex, tb, ncols, col);
+    } else {
+      // the block's 32-bit words, column-sorted (a packed block: the
+      // launcher's condition), at the slot in exception word 1
+      const int slot = __builtin_amdgcn_readlane((int)ex, 1);
+      const int4 bq = base[slot];
+      const int4 bb = make_int4(uniform(bq.x), uniform(bq.y), uniform(bq.z), uniform(bq.w));
+      const uint32_t *w0 = pk + (size_t)slot * stride;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = wv * kDWave + j * 64 + lane;
+        const uint32_t w = e < n ? __builtin_nontemporal_load(w0 + e) : 0u;
+        const uint32_t c = (uint32_t)slot_col(w, bb, true);
+        col[j] = c < (uint32_t)ncols ? c : (uint32_t)(ncols - 1);
+        const uint32_t lo = w & ((1u << (kPackHeadBit + 1)) - 1u);  // position | head bit
+        if (j & 1) w2[j >> 1] |= lo << 16;
+        else w2[j >> 1] = lo;
+      }
+    }
+  }
+  T x[8][KT];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+#pragma unroll
+    for (int f = 0; f < KT; ++f) x[j][f] = q[(size_t)col[j] * KT + f];
+  const int i0 = wv * kDWave + lane;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (i0 + j * 64 < n) {
+      const uint32_t wj = w2[j >> 1] >> ((j & 1) * 16);
+      const int ps = (int)(wj & kPackPosMask);
+      uint8_t *g = stage + (ps >> 3) * G;
+#pragma unroll
+      for (int f = 0; f < KT; ++f) reinterpret_cast<L *>(g)[8 * f + (ps & 7)] = Lds<T>::in(x[j][f]);
+      g[HV + (ps & 7)] = (uint8_t)((wj >> kPackHeadBit) & 1u);
+    }
+  }
+  __syncthreads();
+  // head bits of positions 8t - 8 .. 8t + 15 (bit k: position 8t - 8 + k),
+  // positions from n on (and before 0) counted as heads
+  const int t = tid, P = 8 * t;
+  const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
+  uint32_t bits = head_bits8(*reinterpret_cast<const uint2 *>(stage + gp * G + HV)) |
+                  (head_bits8(*reinterpret_cast<const uint2 *>(stage + t * G + HV)) << 8) |
+                  (head_bits8(*reinterpret_cast<const uint2 *>(stage + gn * G + HV)) << 16);
+  if (t == 0) bits |= 0xffu;
+  const int live = n - (P - 8);
+  if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
+  L res[KT][8];
+#pragma unroll
+  for (int f = 0; f < KT; ++f) segk_fold<T, OP, KT>(stage, f, bits, res[f]);
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < KT; ++f)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) reinterpret_cast<L *>(stage + t * G)[8 * f + k] = res[f][k];
+  __syncthreads();
+  // positions again from w2 (the opaque copy keeps the compiler from holding
+  // the staging addresses across the reduction)
+  uint32_t w3[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    w3[k] = w2[k];
+    asm volatile("" : "+v"(w3[k]));
+  }
+  L y[8][KT];  // every position is in the stage: read all, then store
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int ps = (int)((w3[j >> 1] >> ((j & 1) * 16)) & kPackPosMask);
+    const L *g = reinterpret_cast<const L *>(stage + (ps >> 3) * G) + (ps & 7);
+#pragma unroll
+    for (int f = 0; f < KT; ++f) y[j][f] = g[8 * f];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (i0 + j * 64 < n) {
+#pragma unroll
+      for (int f = 0; f < KT; ++f) q[(size_t)col[j] * KT + f] = Lds<T>::out(y[j][f]);
+    }
+}
+
 // GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
 // the T list alone, on a plan with nMask: each entry's N membership from the
 // column mask (a 4-byte word per 32 columns, cache resident), the T pack's
@@ -3552,6 +3735,43 @@ static int launch_delta(int op, const Plan *p, T *v, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+// The one-list GatherScatter with three fields over the byte codes (the
+// bundled blocks where the plan has them, else partition 0): for lists whose
+// rows have at most 8 members, with no block left in CSR order, and blocks
+// enough to fill the device (else null).
+static const DeltaPack *byte_kv_pack(const Plan *p, const Partition **pt) {
+  const bool bun = p->bundle.part.d != nullptr;
+  const Partition &x = bun ? p->bundle.part : p->partT[0];
+  const DeltaPack &d = bun ? p->bundle.dp : p->dpack;
+  if (!d.w || !d.bw || !d.seg8 || d.csr > 0 || x.nblocks < kFillPerCu * p->cus) return nullptr;
+  *pt = &x;
+  return &d;
+}
+
+template <typename T, int OP>
+static void launch_byte_kv_op(const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
+  const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
+  hipLaunchKernelGGL((k_gs_sym_byte_kv<T, OP, 3>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, d.f.pk, d.f.base,
+                     d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E, tb, ncols, v,
+                     (pt.tiled & kTiledGS) ? pt.order : nullptr, reinterpret_cast<const u32x2 *>(d.bw),
+                     reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
+}
+
+template <typename T>
+static int launch_byte_kv(int op, const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_byte_kv_op<T, OGS_ADD>(pt, d, ncols, v, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_byte_kv_op<T, OGS_MUL>(pt, d, ncols, v, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_byte_kv_op<T, OGS_MAX>(pt, d, ncols, v, st); break;
+    case OGS_MIN: launch_byte_kv_op<T, OGS_MIN>(pt, d, ncols, v, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
 template <typename T>
 static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:321-339
@@ -3587,6 +3807,14 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   if (sym && vec_k(K)) {
     const Partition *vp = vec_launch_partition<T>(p, K);
     if (vp && K == 1 && vp == &p->partT[0] && p->dpack.w) return launch_delta<T>(op, p, v, st);
+    if constexpr (kSeg8<T>) {
+      // (the precision-form types stay on the entry-major kernel: their
+      // compensated folds of the neighbouring groups measured slower, bf16
+      // 239 -> 301 us at 48^3 p=7)
+      const Partition *bp = nullptr;
+      const DeltaPack *bd = K == 3 ? byte_kv_pack(p, &bp) : nullptr;
+      if (bd) return launch_byte_kv<T>(op, *bp, *bd, p->Ncols, v, st);
+    }
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
   // Trans over the T list alone, N membership from the column mask: the
@@ -4441,6 +4669,7 @@ static int build_delta_pack(const Plan *p, const Partition &pt, const Pack &k, D
   if (!rc) rc = (int)hipMemcpyAsync(&fb, scratch, sizeof(fb), hipMemcpyDeviceToHost, st);
   if (!rc) rc = (int)hipStreamSynchronize(st);
   dp.fallback = fb;
+  dp.csr = k.plain;  // a block left in CSR order is always a fallback block
   dp.f = k;  // fallback blocks read the partition's pack (slot = block id)
   dp.ownsF = false;
   dp.rows = (int64_t)p->nnzT >= 2 * (int64_t)p->NrowsT;
@@ -5219,6 +5448,8 @@ int ogs_hip_plan_info(ogs_hip_plan_t p, ogs_hip_plan_info_t *info) {
   info->rowMapFallback = p->rowMapT.codes ? p->rowMapT.fallback : 0;
   info->byteItems = p->dpack.bw ? (int32_t)p->dpack.bItems : 0;
   info->byteFallback = p->dpack.bw ? p->dpack.bFallback : 0;
+  const Partition *bp = nullptr;
+  info->byteK3 = byte_kv_pack(p, &bp) != nullptr;
   info->bytes = plan_bytes(p);
   return OGS_SUCCESS;
 }
@@ -5293,6 +5524,8 @@ struct BatchConfig {
                // descriptors on their three-field partition where the plan has one)
   int order;   // 0: items of larger descriptors first, 1: descriptor order
   int fused;   // 1: all field-count classes in one kernel node
+  int plans;   // 1: one launch per descriptor through its plan's own kernels
+               // (the one-list GatherScatter Sym that ogs_hip_gather_scatter runs)
 };
 
 // Field-count classes with a compile-time shape: K = 1, 2, 3, 4, 8, 16.
@@ -5484,7 +5717,8 @@ struct ogs_hip_batch_s {
   int type = 0, op = 0;
   std::vector<ogs_hip_plan_t> plans;  // retained while the batch lives
   ogs_hip::BatchDevTables tables;
-  ogs_hip::BatchConfig cfg = {0, 1, 1};
+  ogs_hip::BatchConfig cfg = {0, 1, 1, 0};
+  int nplans = 0;  // descriptors (the graph's launches when cfg.plans)
   int cus = 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -5506,6 +5740,16 @@ static void free_batch(ogs_hip_batch_s *b) {
   delete b;
 }
 
+// One GatherScatter Sym launch per descriptor, each through its plan's own
+// kernel choice.
+static int launch_plans(int type, int op, const std::vector<ogs_hip_batch_desc_t> &in,
+                        const std::vector<void *> &qs, hipStream_t st) {
+  int rc = 0;
+  for (size_t i = 0; i < in.size() && !rc; ++i)
+    rc = ogs_hip_gather_scatter(in[i].plan, type, op, OGS_SYM, in[i].K, qs[i], st);
+  return rc;
+}
+
 // Times a configuration on scratch buffers: 1 warm-up, kTuneLaunches timed launches.
 constexpr int kTuneLaunches = 5;  // timed launches per candidate and pass
 
@@ -5519,9 +5763,12 @@ static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t>
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
-  if (!rc) rc = launch_batch(type, op, dt, cus, c.fused, st);
+  auto launch = [&]() {
+    return c.plans ? launch_plans(type, op, in, qs, st) : launch_batch(type, op, dt, cus, c.fused, st);
+  };
+  if (!rc) rc = launch();
   if (!rc) rc = (int)hipEventRecord(e0, st);
-  for (int i = 0; i < kTuneLaunches && !rc; ++i) rc = launch_batch(type, op, dt, cus, c.fused, st);
+  for (int i = 0; i < kTuneLaunches && !rc; ++i) rc = launch();
   if (!rc) rc = (int)hipEventRecord(e1, st);
   if (!rc) rc = (int)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -5557,12 +5804,13 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!b) return (int)hipErrorOutOfMemory;
   b->type = type;
   b->op = op;
+  b->nplans = n;
   int dev = 0, cus = 256;
   int rc = (int)hipGetDevice(&dev);
   if (!rc) rc = (int)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus < 1) cus = 1;
   b->cus = cus;
-  BatchConfig best = {0, 1, 1};
+  BatchConfig best = {0, 1, 1, 0};
   const auto t0 = std::chrono::steady_clock::now();
   if (!rc && tune && n > 0) {
     // scratch copies: tuning launches must not touch the caller's data
@@ -5578,7 +5826,8 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
     std::vector<BatchConfig> cands;
     for (int fused = 0; fused <= 1; ++fused)
       for (int narrow = 0; narrow <= (anyKv ? 1 : 0); ++narrow)
-        for (int order = 0; order <= 1; ++order) cands.push_back({narrow, order, fused});
+        for (int order = 0; order <= 1; ++order) cands.push_back({narrow, order, fused, 0});
+    cands.push_back({0, 1, 0, 1});
     std::vector<float> times(cands.size(), 0.f);
     for (int pass = 0; pass < 2 && !rc; ++pass) {
       for (size_t i = 0; i < cands.size() && !rc; ++i) {
@@ -5609,7 +5858,7 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!rc) rc = (int)hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (!rc) rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (!rc) {
-    const int lr = launch_batch(type, op, b->tables, cus, best.fused, cs);
+    const int lr = best.plans ? launch_plans(type, op, in, qs, cs) : launch_batch(type, op, b->tables, cus, best.fused, cs);
     const int er = (int)hipStreamEndCapture(cs, &b->graph);
     rc = lr ? lr : er;
   }
@@ -5646,7 +5895,8 @@ int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info) {
   int nodes = 0, shaped = 0;
   for (int c = 0; c < ogs_hip::kPipeClasses; ++c) shaped += batch->tables.n[c] > 0;
   for (int c = ogs_hip::kPipeClasses; c < ogs_hip::kPipeClasses + 2; ++c) nodes += batch->tables.n[c] > 0;
-  info->nodes = nodes + (batch->cfg.fused ? (shaped > 0) : shaped);
+  info->nodes = batch->cfg.plans ? (int)batch->nplans : nodes + (batch->cfg.fused ? (shaped > 0) : shaped);
+  info->plans = batch->cfg.plans;
   info->narrow = batch->cfg.narrow;
   info->order = batch->cfg.order;
   info->fused = batch->cfg.fused;

@@ -180,7 +180,10 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * chunks that read the int32 map.  byteItems, byteFallback: exception items
  * of the K=1 GatherScatter's one-byte codes over the delta pack's blocks
  * (both 0 and deltaE > 0: the 16-bit words serve), and their blocks that run
- * from the 16-bit words.  bytes:
+ * from the 16-bit words.  byteK3: 1 where the one-list GatherScatter with
+ * three fields of float, double, int32 or int64 runs over those codes'
+ * blocks too (rows of at most 8 members, no block left in CSR order, blocks
+ * enough to fill the device).  bytes:
  * device memory the plan owns (partitions, packs, column maps, block
  * orders), beside the caller's CSR. */
 typedef struct {
@@ -188,7 +191,7 @@ typedef struct {
   int32_t nblocksK1, widthK1, packedK1, plainK1, Ncols, tileD;
   int32_t colsT, colsN, deltaE, deltaFallback, ownerMap, nMask;
   int32_t bundleRy, bundleRz, bundleTy, bundleTz, bundleTileD, bundleFallback;
-  int32_t rowMapE, rowMapFallback, byteItems, byteFallback;
+  int32_t rowMapE, rowMapFallback, byteItems, byteFallback, byteK3;
   int64_t bytes;
 } ogs_hip_plan_info_t;
 int ogs_hip_plan_info(ogs_hip_plan_t plan, ogs_hip_plan_info_t *info);
@@ -224,10 +227,13 @@ int ogs_hip_batch_destroy(ogs_hip_batch_t batch);
  * kernel nodes in the graph, whether K>1 classes use the next narrower
  * block width, item order (0 = larger descriptors first, 1 = descriptor
  * order), whether all field-count classes share one kernel node (fused),
- * and the tuning time / best run. */
+ * the tuning time / best run, and whether the graph launches each
+ * descriptor through its plan's own GatherScatter kernels instead (plans;
+ * a tuning candidate, picked where it ran fastest). */
 typedef struct {
   int32_t items, nodes, narrow, order, fused, candidates;
   float tune_ms, best_us;
+  int32_t plans;
 } ogs_hip_batch_info_t;
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info);
 
