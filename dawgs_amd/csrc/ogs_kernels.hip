@@ -1332,44 +1332,35 @@ __device__ __forceinline__ uint8_t &seg_head(uint8_t *s, int ps) {
 }
 
 // Row results of a block staged in the kSeg layout whose rows have at most 8
-// members.  Thread t owns positions [8t, 8t + 8) (group t); every row
-// touching them lies within groups t - 1 .. t + 1, so the thread reads those
-// three groups and folds in CSR order from the operator's identity: the row
-// open at 8t (begun in group t - 1), its own rows, and its last row on into
-// group t + 1 up to that group's first head.  res[k]: the result of the row
-// holding position 8t + k; bits: head bits of positions 8t - 8 .. 8t + 15
-// (bit k: position 8t - 8 + k), positions from n on counted as heads.
+// members.  Thread t owns positions [8t, 8t + 8) (group t) and folds the
+// rows whose heads lie there, in CSR order from the operator's identity, the
+// last of them on into group t + 1 up to that group's first head (a row of
+// at most 8 members ends there); the row open at 8t is group t - 1's last
+// row, folded by thread t - 1.  So every row is folded once, by the thread
+// holding its head.  res[k]: the result of the row holding position 8t + k,
+// for k from the group's first head on (res[7]: of the row that continues
+// into group t + 1); bits: head bits of positions 8t .. 8t + 15 at bits
+// 8 .. 23 (bit 8 + k: position 8t + k), positions from n on counted as heads.
+// (Folding the open row again from group t - 1 in every thread, as before,
+// took a third more adds and LDS reads.)
 template <typename T, int OP>
 __device__ __forceinline__ void seg8_fold(int n, const uint8_t *__restrict__ s, T (&res)[8], uint32_t &bits) {
   using R = LRed<T, OP>;
   using A = typename R::A;
   constexpr int G = seg_group<T>(), HV = 8 * (int)sizeof(T);
   const int t = (int)threadIdx.x, P = 8 * t;
-  const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
-  const uint8_t *sp = s + gp * G, *so = s + t * G, *sn = s + gn * G;
-  bits = head_bits8(*reinterpret_cast<const uint2 *>(sp + HV)) |
-                  (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
-                  (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
-  if (t == 0) bits |= 0xffu;     // no previous group
+  const int gn = t + 1 < kCap / 8 ? t + 1 : t;
+  const uint8_t *so = s + t * G, *sn = s + gn * G;
+  bits = (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
+         (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
   const int live = n - (P - 8);  // bit k is position P - 8 + k: from bit `live` on, past the block's entries
   if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
-  // three phases, each reading 8 values: held apart (sched_barrier) so that
+  // two phases, each reading 8 values: held apart (sched_barrier) so that
   // no more than 16 are in registers at once
   T x[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(sp)[k];
-  // the row open at P: from group t - 1's last head
-  const int lh = 31 - __builtin_clz(bits & 0xffu);
-  A c = R::init();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    A y = c;
-    R::apply(y, x[k]);
-    c = k >= lh ? y : c;
-  }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
   for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(so)[k];
+  A c = R::init();
   A part[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1396,16 +1387,32 @@ __device__ __forceinline__ void seg8_fold(int n, const uint8_t *__restrict__ s, 
   }
 }
 
-// seg8_fold, each own position then taking its row's result (written after
-// a barrier, since neighbours read the group).  Ends with a barrier.
+// Writes the group-t fold's results over the stage (after a barrier, since
+// neighbours read the group): own positions from the group's first head on,
+// and group t + 1's positions before its first head (the last own row's).
+template <typename L>
+__device__ __forceinline__ void seg8_put(uint8_t *__restrict__ own, uint8_t *__restrict__ next, uint32_t bits,
+                                         const L (&res)[8]) {
+  const int fo = __builtin_ctz((bits >> 8) | 0x100u), fh = __builtin_ctz(bits >> 16);
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k >= fo) reinterpret_cast<L *>(own)[k] = res[k];
+  if ((int)threadIdx.x + 1 < kCap / 8) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if (k < fh) reinterpret_cast<L *>(next)[k] = res[7];
+  }
+}
+
+// seg8_fold, each position then taking its row's result.  Ends with a
+// barrier.
 template <typename T, int OP>
 __device__ __forceinline__ void seg8_reduce(int n, uint8_t *__restrict__ s) {
   T res[8];
   uint32_t bits;
   seg8_fold<T, OP>(n, s, res, bits);
   __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) reinterpret_cast<T *>(s + threadIdx.x * seg_group<T>())[k] = res[k];
+  seg8_put<T>(s + threadIdx.x * seg_group<T>(), s + (threadIdx.x + 1) * seg_group<T>(), bits, res);
   __syncthreads();
 }
 
@@ -1673,24 +1680,30 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte(
 
 // ---------------------------------------------------------------------------
 // The byte pack's blocks with KT interleaved fields: the one-list
-// GatherScatter Sym at K = KT on lists whose rows have at most 8 members.
+// GatherScatter Sym at K = KT on lists whose rows have at most 8 members
+// (the reference's four types and float-compensated storage).
 // One decode of a block's codes (the K = 1 kernel's) feeds KT contiguous
 // value loads per entry, so the index costs ~1 byte per entry instead of the
 // 4-byte pack word of the entry-major kernel, whose blocks hold only as many
 // entries as 16 KB of values (682 at K = 3 in fp64).  The stage holds the
 // block's kCap / 8 groups of 8 CSR positions, each group's KT x 8 values
-// (field-major) followed by its 8 head bytes; thread t folds group t for
-// every field (rows read from groups t - 1 .. t + 1, as seg8_fold), holds
-// the results until every thread has read the stage, and writes them back.
+// (field-major) followed by its 8 head bytes; thread t folds the rows whose
+// heads lie in group t for every field (as seg8_fold), holds the results
+// until every thread has read the stage, and writes them back.
 // Blocks without codes run from their 16-bit words, and blocks without those
 // from their 32-bit words: the launcher runs this kernel only on packs that
 // left no block in CSR order.
 template <typename L, int KT>
 __host__ __device__ constexpr int segk_group() { return KT * 8 * (int)sizeof(L) + 8; }
 
-// Rows of the group-t fold of field f (bits as seg8_fold builds them); res[k]
-// is the result of the row holding position 8t + k.
-template <typename T, int OP, int KT>
+// The group-t fold of field f (seg8_fold's rows and bits): res[k] is the
+// result of the row holding position 8t + k, from the group's first head on;
+// with OPEN also before it, the row open at 8t folded again from group
+// t - 1's last head (bits 0 .. 7: group t - 1's head bits, all set for
+// t = 0).  OPEN serves the plain types' three-field kernel (fp64 K = 3 at
+// 48^3 p=7: 560 us, against 572 us folding each row once); the compensated
+// sums, whose adds cost more, fold each row once.
+template <typename T, int OP, int KT, bool OPEN>
 __device__ __forceinline__ void segk_fold(const uint8_t *__restrict__ s, int f, uint32_t bits,
                                           typename Lds<T>::L (&res)[8]) {
   using L = typename Lds<T>::L;
@@ -1703,18 +1716,21 @@ __device__ __forceinline__ void segk_fold(const uint8_t *__restrict__ s, int f, 
   const L *so = reinterpret_cast<const L *>(s + t * G) + 8 * f;
   const L *sn = reinterpret_cast<const L *>(s + gn * G) + 8 * f;
   L x[8];
+  if constexpr (OPEN) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) x[k] = sp[k];
-  // the row open at 8t: from group t - 1's last head
-  const int lh = 31 - __builtin_clz(bits & 0xffu);
-  A c = R::init();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    A y = c;
-    R::apply(y, x[k]);
-    c = k >= lh ? y : c;
+    for (int k = 0; k < 8; ++k) x[k] = sp[k];
   }
-  __builtin_amdgcn_sched_barrier(0);
+  A c = R::init();
+  if constexpr (OPEN) {
+    const int lh = 31 - __builtin_clz(bits & 0xffu);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      A y = c;
+      R::apply(y, x[k]);
+      c = k >= lh ? y : c;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) x[k] = so[k];
   A part[8];
@@ -1813,22 +1829,31 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
   __syncthreads();
   // head bits of positions 8t - 8 .. 8t + 15 (bit k: position 8t - 8 + k),
   // positions from n on (and before 0) counted as heads
+  constexpr bool OPEN = !std::is_same<typename LRed<T, OP>::A, Comp>::value;
   const int t = tid, P = 8 * t;
   const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
-  uint32_t bits = head_bits8(*reinterpret_cast<const uint2 *>(stage + gp * G + HV)) |
-                  (head_bits8(*reinterpret_cast<const uint2 *>(stage + t * G + HV)) << 8) |
+  uint32_t bits = (head_bits8(*reinterpret_cast<const uint2 *>(stage + t * G + HV)) << 8) |
                   (head_bits8(*reinterpret_cast<const uint2 *>(stage + gn * G + HV)) << 16);
-  if (t == 0) bits |= 0xffu;
+  if constexpr (OPEN) {
+    bits |= head_bits8(*reinterpret_cast<const uint2 *>(stage + gp * G + HV));
+    if (t == 0) bits |= 0xffu;
+  }
   const int live = n - (P - 8);
   if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
   L res[KT][8];
 #pragma unroll
-  for (int f = 0; f < KT; ++f) segk_fold<T, OP, KT>(stage, f, bits, res[f]);
+  for (int f = 0; f < KT; ++f) segk_fold<T, OP, KT, OPEN>(stage, f, bits, res[f]);
   __syncthreads();
 #pragma unroll
-  for (int f = 0; f < KT; ++f)
+  for (int f = 0; f < KT; ++f) {
+    if constexpr (OPEN) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) reinterpret_cast<L *>(stage + t * G)[8 * f + k] = res[f][k];
+      for (int k = 0; k < 8; ++k) reinterpret_cast<L *>(stage + t * G)[8 * f + k] = res[f][k];
+    } else {
+      seg8_put<L>(stage + t * G + f * 8 * (int)sizeof(L), stage + (t + 1) * G + f * 8 * (int)sizeof(L), bits,
+                  res[f]);
+    }
+  }
   __syncthreads();
   // positions again from w2 (the opaque copy keeps the compiler from holding
   // the staging addresses across the reduction)
@@ -3807,10 +3832,10 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
   if (sym && vec_k(K)) {
     const Partition *vp = vec_launch_partition<T>(p, K);
     if (vp && K == 1 && vp == &p->partT[0] && p->dpack.w) return launch_delta<T>(op, p, v, st);
-    if constexpr (kSeg8<T>) {
-      // (the precision-form types stay on the entry-major kernel: their
-      // compensated folds of the neighbouring groups measured slower, bf16
-      // 239 -> 301 us at 48^3 p=7)
+    if constexpr (kSeg8<T> || std::is_same<T, f32c>::value) {
+      // (float-compensated K = 3 at 48^3 p=7: 314 -> 290 us, 192^3 p=1:
+      // 579 -> 328 us; bfloat16, whose values are widened, stays on the
+      // entry-major kernel: 253 -> 282 us at 48^3 p=7)
       const Partition *bp = nullptr;
       const DeltaPack *bd = K == 3 ? byte_kv_pack(p, &bp) : nullptr;
       if (bd) return launch_byte_kv<T>(op, *bp, *bd, p->Ncols, v, st);

@@ -1,11 +1,10 @@
 """GPU parity of the one-list GatherScatter Sym with three interleaved fields
 over the K = 1 kernel's one-byte codes (k_gs_sym_byte_kv): bit-exact
-against the oracle for the reference's four types and every op on meshes
-whose partition 0 fills the device, for blocks that fall back to the 16-bit
-and 32-bit words, on bundled low-degree blocks and on Signed operators.  The
-precision-form storage types (float-compensated, bfloat16), and lists with
-rows of more than 8 members, keep the entry-major kernel: checked on the
-same meshes.
+against the oracle for the reference's four types and every op, and for
+float-compensated storage, on meshes whose partition 0 fills the device, for
+blocks that fall back to the 16-bit and 32-bit words, on bundled low-degree
+blocks and on Signed operators.  bfloat16 storage, and lists with rows of
+more than 8 members, keep the entry-major kernel: checked on the same meshes.
 """
 import numpy as np
 import pytest
@@ -93,7 +92,9 @@ def test_byte_k3_k1_plan(torch, D):
 
 
 @pytest.mark.parametrize("ty,op", [(O.FLOAT_C, 0), (O.FLOAT_C, 2), (O.BF16, 0), (O.BF16, 3)])
-def test_k3_precision_types_on_byte_meshes(torch, D, ty, op):
+def test_byte_k3_precision_types(torch, D, ty, op):
+    """Float-compensated storage runs the byte kernel (each row folded once,
+    by the thread holding its head); bfloat16 the entry-major kernel."""
     g, ref, N = mesh(D, (16, 16, 16, 7))
     rng = np.random.default_rng(50 + ty + op)
     f = (rng.uniform(-1, 1, K * N) * 10.0 ** rng.uniform(-3, 3, K * N)).astype(np.float32)
