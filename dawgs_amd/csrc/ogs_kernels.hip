@@ -1335,32 +1335,55 @@ __device__ __forceinline__ uint8_t &seg_head(uint8_t *s, int ps) {
 // members.  Thread t owns positions [8t, 8t + 8) (group t) and folds the
 // rows whose heads lie there, in CSR order from the operator's identity, the
 // last of them on into group t + 1 up to that group's first head (a row of
-// at most 8 members ends there); the row open at 8t is group t - 1's last
-// row, folded by thread t - 1.  So every row is folded once, by the thread
-// holding its head.  res[k]: the result of the row holding position 8t + k,
-// for k from the group's first head on (res[7]: of the row that continues
-// into group t + 1); bits: head bits of positions 8t .. 8t + 15 at bits
-// 8 .. 23 (bit 8 + k: position 8t + k), positions from n on counted as heads.
-// (Folding the open row again from group t - 1 in every thread, as before,
-// took a third more adds and LDS reads.)
-template <typename T, int OP>
+// at most 8 members ends there).  res[k]: the result of the row holding
+// position 8t + k, for k from the group's first head on (res[7]: of the row
+// that continues into group t + 1); bits: head bits of positions 8t .. 8t +
+// 15 at bits 8 .. 23 (bit 8 + k: position 8t + k), positions from n on
+// counted as heads.  With OPEN the thread also folds the row open at 8t
+// again, from group t - 1's last head (bits 0 .. 7: group t - 1's head bits,
+// all set for t = 0), so that res covers all 8 positions: the GatherScatter
+// writes each position from its own thread that way (one fold per row
+// instead measured 1% slower at 96^3 p=3 and 192^3 p=1, equal at 48^3 p=7);
+// the Gather needs only the heads' results (one fold per row: 48^3 p=7
+// 159.3 -> 156.9 us).
+template <typename T, int OP, bool OPEN>
 __device__ __forceinline__ void seg8_fold(int n, const uint8_t *__restrict__ s, T (&res)[8], uint32_t &bits) {
   using R = LRed<T, OP>;
   using A = typename R::A;
   constexpr int G = seg_group<T>(), HV = 8 * (int)sizeof(T);
   const int t = (int)threadIdx.x, P = 8 * t;
-  const int gn = t + 1 < kCap / 8 ? t + 1 : t;
-  const uint8_t *so = s + t * G, *sn = s + gn * G;
-  bits = (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
-         (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
+  const int gp = t > 0 ? t - 1 : t, gn = t + 1 < kCap / 8 ? t + 1 : t;
+  const uint8_t *sp = s + gp * G, *so = s + t * G, *sn = s + gn * G;
+  if constexpr (OPEN) {
+    bits = head_bits8(*reinterpret_cast<const uint2 *>(sp + HV)) |
+           (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
+           (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
+    if (t == 0) bits |= 0xffu;  // no previous group
+  } else {
+    bits = (head_bits8(*reinterpret_cast<const uint2 *>(so + HV)) << 8) |
+           (head_bits8(*reinterpret_cast<const uint2 *>(sn + HV)) << 16);
+  }
   const int live = n - (P - 8);  // bit k is position P - 8 + k: from bit `live` on, past the block's entries
   if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
-  // two phases, each reading 8 values: held apart (sched_barrier) so that
-  // no more than 16 are in registers at once
+  // phases of 8 values each: held apart (sched_barrier) so that no more
+  // than 16 are in registers at once
   T x[8];
+  A c = R::init();
+  if constexpr (OPEN) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(sp)[k];
+    // the row open at P: from group t - 1's last head
+    const int lh = 31 - __builtin_clz(bits & 0xffu);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      A y = c;
+      R::apply(y, x[k]);
+      c = k >= lh ? y : c;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) x[k] = reinterpret_cast<const T *>(so)[k];
-  A c = R::init();
   A part[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1387,9 +1410,10 @@ __device__ __forceinline__ void seg8_fold(int n, const uint8_t *__restrict__ s, 
   }
 }
 
-// Writes the group-t fold's results over the stage (after a barrier, since
-// neighbours read the group): own positions from the group's first head on,
-// and group t + 1's positions before its first head (the last own row's).
+// Writes a once-per-row group fold's results over the stage (after a
+// barrier, since neighbours read the group): own positions from the group's
+// first head on, and group t + 1's positions before its first head (the
+// last own row's).
 template <typename L>
 __device__ __forceinline__ void seg8_put(uint8_t *__restrict__ own, uint8_t *__restrict__ next, uint32_t bits,
                                          const L (&res)[8]) {
@@ -1404,15 +1428,16 @@ __device__ __forceinline__ void seg8_put(uint8_t *__restrict__ own, uint8_t *__r
   }
 }
 
-// seg8_fold, each position then taking its row's result.  Ends with a
-// barrier.
+// seg8_fold (OPEN), each own position then taking its row's result (written
+// after a barrier, since neighbours read the group).  Ends with a barrier.
 template <typename T, int OP>
 __device__ __forceinline__ void seg8_reduce(int n, uint8_t *__restrict__ s) {
   T res[8];
   uint32_t bits;
-  seg8_fold<T, OP>(n, s, res, bits);
+  seg8_fold<T, OP, true>(n, s, res, bits);
   __syncthreads();
-  seg8_put<T>(s + threadIdx.x * seg_group<T>(), s + (threadIdx.x + 1) * seg_group<T>(), bits, res);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) reinterpret_cast<T *>(s + threadIdx.x * seg_group<T>())[k] = res[k];
   __syncthreads();
 }
 
@@ -1428,7 +1453,7 @@ __device__ __forceinline__ void seg8_gather(int n, int r0, uint8_t *__restrict__
                                             T *__restrict__ gq) {
   T res[8];
   uint32_t bits;
-  seg8_fold<T, OP>(n, s, res, bits);
+  seg8_fold<T, OP, false>(n, s, res, bits);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int own = n - 8 * (int)threadIdx.x;  // own positions below n
   const uint32_t hh = ((bits >> 8) & 0xffu) & (own >= 8 ? 0xffu : own > 0 ? (1u << own) - 1u : 0u);
@@ -1759,22 +1784,19 @@ __device__ __forceinline__ void segk_fold(const uint8_t *__restrict__ s, int f, 
   }
 }
 
-template <typename T, int OP, int KT>
-__global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
-    const int4 *__restrict__ part, const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride,
-    const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q,
-    const int *__restrict__ order, const u32x2 *__restrict__ bw, const uint4 *__restrict__ bh,
-    const u32x2 *__restrict__ bx, const uint32_t *__restrict__ bt) {
-  using L = typename Lds<T>::L;
-  constexpr int G = segk_group<L, KT>(), HV = KT * 8 * (int)sizeof(L);
-  __shared__ __align__(16) uint8_t stage[(kCap / 8) * G];
-  __shared__ uint32_t tab[256];
+// Columns (col) and CSR positions with head bits (w2, as byte_columns) of
+// block b's slots, from its byte codes, else its 16-bit words, else its
+// 32-bit words (packed: the launchers' condition) at the slot in exception
+// word 1.  tab: the block's 256-word LDS table.
+__device__ __forceinline__ void byte_kv_decode(int b, int n, const uint32_t *__restrict__ pk,
+                                               const int4 *__restrict__ base, int stride,
+                                               const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx, int E,
+                                               uint4 tb, int ncols, const u32x2 *__restrict__ bw,
+                                               const uint4 *__restrict__ bh, const u32x2 *__restrict__ bx,
+                                               const uint32_t *__restrict__ bt, uint32_t *__restrict__ tab,
+                                               uint32_t (&col)[8], uint32_t (&w2)[4]) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int xb = xcd_block(blockIdx.x, gridDim.x);
-  const int b = order ? order[xb] : xb;
-  const int n = uniform(part[b + 1].y) - uniform(part[b].y);
   const uint4 h = bh[b];
-  uint32_t col[8], w2[4];
   if (uniform((int)h.x) != (int)kDFallback) {
     const u32x2 cw = __builtin_nontemporal_load(bw + (size_t)b * kBlock + tid);
     tab[tid] = bt[tid];
@@ -1809,6 +1831,16 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
       }
     }
   }
+}
+
+// The block's 3 fields per slot loaded and staged as groups of 8 positions
+// (KT x 8 values, field-major, then the 8 head bytes); ends with a barrier.
+template <typename T, int KT>
+__device__ __forceinline__ void byte_kv_stage(const uint32_t (&col)[8], const uint32_t (&w2)[4], int n,
+                                              const T *__restrict__ q, uint8_t *__restrict__ stage) {
+  using L = typename Lds<T>::L;
+  constexpr int G = segk_group<L, KT>(), HV = KT * 8 * (int)sizeof(L);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   T x[8][KT];
 #pragma unroll
   for (int j = 0; j < 8; ++j)
@@ -1827,6 +1859,26 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
     }
   }
   __syncthreads();
+}
+
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
+    const int4 *__restrict__ part, const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride,
+    const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols, T *__restrict__ q,
+    const int *__restrict__ order, const u32x2 *__restrict__ bw, const uint4 *__restrict__ bh,
+    const u32x2 *__restrict__ bx, const uint32_t *__restrict__ bt) {
+  using L = typename Lds<T>::L;
+  constexpr int G = segk_group<L, KT>(), HV = KT * 8 * (int)sizeof(L);
+  __shared__ __align__(16) uint8_t stage[(kCap / 8) * G];
+  __shared__ uint32_t tab[256];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const int n = uniform(part[b + 1].y) - uniform(part[b].y);
+  uint32_t col[8], w2[4];
+  byte_kv_decode(b, n, pk, base, stride, dw, dx, E, tb, ncols, bw, bh, bx, bt, tab, col, w2);
+  byte_kv_stage<T, KT>(col, w2, n, q, stage);
+  const int i0 = wv * kDWave + lane;
   // head bits of positions 8t - 8 .. 8t + 15 (bit k: position 8t - 8 + k),
   // positions from n on (and before 0) counted as heads
   constexpr bool OPEN = !std::is_same<typename LRed<T, OP>::A, Comp>::value;
@@ -1877,6 +1929,64 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
 #pragma unroll
       for (int f = 0; f < KT; ++f) q[(size_t)col[j] * KT + f] = Lds<T>::out(y[j][f]);
     }
+}
+
+// Gather with KT fields over the byte pack's blocks (Trans/Sym, and NoTrans
+// of an Unsigned operator; rows of at most 8 members): the three-field
+// kernel's decode and stage, each row folded once by the thread holding its
+// head, the block's rows' results put in row order in the stage (row index
+// = heads before it: per thread, a wave scan, the waves before through
+// wsum) and stored as one contiguous run, gq[(r0 + row) * KT + f].
+template <typename T, int OP, int KT>
+__global__ __launch_bounds__(kBlock) void k_gather_byte_kv(
+    const int4 *__restrict__ part, const uint32_t *__restrict__ pk, const int4 *__restrict__ base, int stride,
+    const u32x4 *__restrict__ dw, const uint32_t *__restrict__ dx, int E, uint4 tb, int ncols,
+    const T *__restrict__ q, T *__restrict__ gq, const int *__restrict__ order, const u32x2 *__restrict__ bw,
+    const uint4 *__restrict__ bh, const u32x2 *__restrict__ bx, const uint32_t *__restrict__ bt) {
+  using L = typename Lds<T>::L;
+  constexpr int G = segk_group<L, KT>(), HV = KT * 8 * (int)sizeof(L);
+  static_assert(kCap * KT * (int)sizeof(L) <= (kCap / 8) * G, "a block's row results fit the stage");
+  __shared__ __align__(16) uint8_t stage[(kCap / 8) * G];
+  __shared__ uint32_t tab[256];
+  __shared__ uint32_t wsum[kDRanges];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int xb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = order ? order[xb] : xb;
+  const int r0 = uniform(part[b].x);
+  const int n = uniform(part[b + 1].y) - uniform(part[b].y);
+  uint32_t col[8], w2[4];
+  byte_kv_decode(b, n, pk, base, stride, dw, dx, E, tb, ncols, bw, bh, bx, bt, tab, col, w2);
+  byte_kv_stage<T, KT>(col, w2, n, q, stage);
+  // head bits of positions 8t .. 8t + 15 at bits 8 .. 23, positions from n
+  // on counted as heads
+  const int t = tid, P = 8 * t;
+  const int gn = t + 1 < kCap / 8 ? t + 1 : t;
+  uint32_t bits = (head_bits8(*reinterpret_cast<const uint2 *>(stage + t * G + HV)) << 8) |
+                  (head_bits8(*reinterpret_cast<const uint2 *>(stage + gn * G + HV)) << 16);
+  const int live = n - (P - 8);
+  if (live < 24) bits |= ~0u << (live > 0 ? live : 0);
+  L res[KT][8];
+#pragma unroll
+  for (int f = 0; f < KT; ++f) segk_fold<T, OP, KT, false>(stage, f, bits, res[f]);
+  const int own = n - P;  // own positions below n
+  const uint32_t hh = ((bits >> 8) & 0xffu) & (own >= 8 ? 0xffu : own > 0 ? (1u << own) - 1u : 0u);
+  const uint32_t hc = (uint32_t)__popc(hh);
+  const uint32_t inc = wave_scan_incl(hc);
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();  // (also: every thread's reads of the stage are done)
+  uint32_t row = inc - hc;
+  for (int w = 0; w < wv; ++w) row += wsum[w];
+  const int rows = (int)(wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+  L *rb = reinterpret_cast<L *>(stage);  // rows <= n <= kCap results of KT fields
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if ((hh >> k) & 1u) {
+      const int r = (int)row + __popc(hh & ((1u << k) - 1u));
+#pragma unroll
+      for (int f = 0; f < KT; ++f) rb[r * KT + f] = res[f][k];
+    }
+  __syncthreads();
+  for (int i = tid; i < rows * KT; i += kBlock) gq[(size_t)r0 * KT + i] = Lds<T>::out(rb[i]);
 }
 
 // GatherScatter Trans (gather T, scatter N; ogsOperator.cpp:321-339) over
@@ -4013,6 +4123,39 @@ static int launch_gather_delta(int op, const Plan *p, const T *v, T *gv, hipStre
   return (int)hipGetLastError();
 }
 
+// Gather with three fields over partition 0's byte codes (rows of at most 8
+// members, no block left in CSR order, blocks enough to fill the device).
+static bool gather_kv_ready(const Plan *p) {
+  const DeltaPack &d = p->dpack;
+  return d.w && d.bw && d.seg8 && d.csr == 0 && p->partT[0].nblocks >= kFillPerCu * p->cus;
+}
+
+template <typename T, int OP>
+static void launch_gather_kv_op(const Plan *p, const T *v, T *gv, hipStream_t st) {
+  const Partition &pt = p->partT[0];
+  const DeltaPack &d = p->dpack;
+  const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
+  hipLaunchKernelGGL((k_gather_byte_kv<T, OP, 3>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, d.f.pk, d.f.base,
+                     d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E, tb, p->Ncols, v, gv,
+                     (pt.tiled & kTiledGather) ? pt.order : nullptr, reinterpret_cast<const u32x2 *>(d.bw),
+                     reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
+}
+
+template <typename T>
+static int launch_gather_kv(int op, const Plan *p, const T *v, T *gv, hipStream_t st) {
+  switch (op) {
+    case OGS_ADD: launch_gather_kv_op<T, OGS_ADD>(p, v, gv, st); break;
+    case OGS_MUL:
+      if constexpr (kOffered<T, OGS_MUL>) launch_gather_kv_op<T, OGS_MUL>(p, v, gv, st);
+      else return OGS_E_NOT_SUPPORTED;
+      break;
+    case OGS_MAX: launch_gather_kv_op<T, OGS_MAX>(p, v, gv, st); break;
+    case OGS_MIN: launch_gather_kv_op<T, OGS_MIN>(p, v, gv, st); break;
+    default: return OGS_E_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
 template <typename T>
 static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_gv, const void *d_v, hipStream_t st) {
   // list selection: ogsOperator.cpp:71-79
@@ -4026,6 +4169,10 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
     const Partition *vp = vec_launch_partition<T>(p, K);
     if (vp && K == 1 && vp == &p->partT[0] && p->dpack.w)
       return launch_gather_delta<T>(op, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv), st);
+    if constexpr (kSeg8<T> || std::is_same<T, f32c>::value) {
+      if (K == 3 && gather_kv_ready(p))
+        return launch_gather_kv<T>(op, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv), st);
+    }
     if (vp) {
       const ListView G = list_view(p, false, vp, false);
       const int *order = (K == 1 && (vp->tiled & kTiledGather)) ? vp->order : nullptr;

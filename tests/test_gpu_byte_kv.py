@@ -1,5 +1,6 @@
-"""GPU parity of the one-list GatherScatter Sym with three interleaved fields
-over the K = 1 kernel's one-byte codes (k_gs_sym_byte_kv): bit-exact
+"""GPU parity of the one-list GatherScatter Sym and the Gather with three
+interleaved fields over the K = 1 kernel's one-byte codes (k_gs_sym_byte_kv,
+k_gather_byte_kv; the Gather on partition 0, never on bundled blocks): bit-exact
 against the oracle for the reference's four types and every op, and for
 float-compensated storage, on meshes whose partition 0 fills the device, for
 blocks that fall back to the 16-bit and 32-bit words, on bundled low-degree
@@ -181,3 +182,50 @@ def test_rows_over_8_keep_entry_major(torch, D):
     assert info["maxRowT"] > 8 and info["byteK3"] == 0, info
     got, want = run(torch, D, g, ref, N, np.float64, O.ADD, 41)
     assert same(got, want)
+
+
+def run_gather(torch, D, g, ref, N, dt, op, seed, ty=None):
+    rng = np.random.default_rng(seed)
+    v = values(rng, dt, K * N) if ty is None else \
+        (rng.uniform(-1, 1, K * N) * 10.0 ** rng.uniform(-3, 3, K * N)).astype(np.float32)
+    dv = torch.from_numpy(v.copy()).cuda()
+    for trans in (O.TRANS, O.NOTRANS):
+        want = ref.gather(v, K, op, trans, type=ty)
+        gq = torch.zeros(len(want), dtype=dv.dtype, device="cuda")
+        g.plan.Gather(gq, dv, K, op, trans, type=(D.FloatC if ty == O.FLOAT_C else None))
+        torch.cuda.synchronize()
+        assert same(gq.cpu().numpy(), want), (np.dtype(dt).name, op, trans)
+
+
+@pytest.mark.parametrize("box", BOXES, ids=lambda b: "x".join(map(str, b)))
+@pytest.mark.parametrize("ty", list(TYPES))
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_byte_gather_k3(torch, D, box, ty, op):
+    """Gather Trans and NoTrans (the Unsigned operator's lists alias) with
+    three fields, each row's results at its row."""
+    g, ref, N = mesh(D, box)
+    run_gather(torch, D, g, ref, N, TYPES[ty], op, hash((box, ty, op, "g3")) % 2 ** 32)
+
+
+@pytest.mark.parametrize("op", [0, 2])
+def test_byte_gather_k3_float_c(torch, D, op):
+    g, ref, N = mesh(D, (16, 16, 16, 7))
+    run_gather(torch, D, g, ref, N, np.float32, op, 60 + op, ty=O.FLOAT_C)
+
+
+@pytest.mark.parametrize("fields", [0, 1], ids=["all_k", "k1_plan"])
+def test_byte_gather_k3_fallback_blocks(torch, D, fields):
+    ids = shuffled((36, 32, 32, 3), 0.08, 3)
+    g, ref, N = mesh(D, None, fields=fields, ids=ids, key=("shuffled", fields))
+    info = g.plan.info()
+    if info["plainK1"] > 0:
+        pytest.skip(f"blocks left in CSR order: the entry-major kernel serves ({info})")
+    for dt, op in ((np.float64, 0), (np.int64, 3)):
+        run_gather(torch, D, g, ref, N, dt, op, 70 + op)
+
+
+def test_byte_gather_k3_bundled_plan(torch, D):
+    """A plan with bundled GatherScatter blocks gathers over partition 0."""
+    g, ref, N = mesh(D, (128, 128, 128, 2), fields=D.PlanK1)
+    assert g.plan.info()["bundleTy"] > 0
+    run_gather(torch, D, g, ref, N, np.float64, O.ADD, 80)
