@@ -5890,7 +5890,8 @@ struct ogs_hip_batch_s {
   std::vector<ogs_hip_plan_t> plans;  // retained while the batch lives
   ogs_hip::BatchDevTables tables;
   ogs_hip::BatchConfig cfg = {0, 1, 1, 0};
-  int nplans = 0;  // descriptors (the graph's launches when cfg.plans)
+  int nplans = 0;     // descriptors launched through their plans (cfg.plans)
+  int planItems = 0;  // their blocks (partition 0 or the bundles)
   int cus = 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -5922,22 +5923,47 @@ static int launch_plans(int type, int op, const std::vector<ogs_hip_batch_desc_t
   return rc;
 }
 
+// The descriptors a configuration launches through their plans (P) and
+// through the batched kernels (B).  (Per-plan launches for the descriptors
+// whose plans run byte-coded kernels, the rest batched, was raced too and
+// never picked on the 48-mesh batch: 0.98 ms either way.)
+struct BatchSplit {
+  std::vector<ogs_hip_batch_desc_t> inP, inB;
+  std::vector<void *> qsP, qsB;
+};
+static void split_batch(const BatchConfig &c, const std::vector<ogs_hip_batch_desc_t> &in,
+                        const std::vector<void *> &qs, BatchSplit &sp) {
+  for (size_t i = 0; i < in.size(); ++i) {
+    const bool viaPlan = c.plans == 1;
+    (viaPlan ? sp.inP : sp.inB).push_back(in[i]);
+    (viaPlan ? sp.qsP : sp.qsB).push_back(qs[i]);
+  }
+}
+
+// One run of a configuration: the per-plan launches, then the batched kernels.
+static int launch_split(int type, int op, const BatchSplit &sp, const BatchDevTables &dt, int cus, int fused,
+                        hipStream_t st) {
+  int rc = sp.inP.empty() ? 0 : launch_plans(type, op, sp.inP, sp.qsP, st);
+  if (!rc && !sp.inB.empty()) rc = launch_batch(type, op, dt, cus, fused, st);
+  return rc;
+}
+
 // Times a configuration on scratch buffers: 1 warm-up, kTuneLaunches timed launches.
 constexpr int kTuneLaunches = 5;  // timed launches per candidate and pass
 
 static int time_config(int type, int op, const std::vector<ogs_hip_batch_desc_t> &in, const std::vector<void *> &qs,
                        const BatchConfig &c, int cus, hipStream_t st, float *us) {
+  BatchSplit sp;
+  split_batch(c, in, qs, sp);
   BatchTables t;
-  int rc = make_tables(type, in, qs, c, t);
+  int rc = make_tables(type, sp.inB, sp.qsB, c, t);
   if (rc) return rc;
   BatchDevTables dt;
   rc = upload_tables(t, st, dt);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = (int)hipEventCreate(&e0);
   if (!rc) rc = (int)hipEventCreate(&e1);
-  auto launch = [&]() {
-    return c.plans ? launch_plans(type, op, in, qs, st) : launch_batch(type, op, dt, cus, c.fused, st);
-  };
+  auto launch = [&]() { return launch_split(type, op, sp, dt, cus, c.fused, st); };
   if (!rc) rc = launch();
   if (!rc) rc = (int)hipEventRecord(e0, st);
   for (int i = 0; i < kTuneLaunches && !rc; ++i) rc = launch();
@@ -5976,7 +6002,6 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   if (!b) return (int)hipErrorOutOfMemory;
   b->type = type;
   b->op = op;
-  b->nplans = n;
   int dev = 0, cus = 256;
   int rc = (int)hipGetDevice(&dev);
   if (!rc) rc = (int)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -6023,14 +6048,21 @@ int ogs_hip_batch_create(ogs_hip_batch_t *batch, int type, int op, const ogs_hip
   b->tune_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   b->cfg = best;
   BatchTables t;
-  if (!rc) rc = make_tables(type, in, qs, best, t);
+  BatchSplit sp;
+  split_batch(best, in, qs, sp);
+  b->nplans = (int)sp.inP.size();
+  for (const ogs_hip_batch_desc_t &d : sp.inP) {
+    const Plan *p = d.plan;
+    b->planItems += p->bundle.part.d ? p->bundle.part.nblocks : p->nparts ? p->partT[0].nblocks : 0;
+  }
+  if (!rc) rc = make_tables(type, sp.inB, sp.qsB, best, t);
   if (!rc) rc = upload_tables(t, st, b->tables);
   // capture the launch in a graph (capture needs a stream of its own)
   hipStream_t cs = nullptr;
   if (!rc) rc = (int)hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (!rc) rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (!rc) {
-    const int lr = best.plans ? launch_plans(type, op, in, qs, cs) : launch_batch(type, op, b->tables, cus, best.fused, cs);
+    const int lr = launch_split(type, op, sp, b->tables, cus, best.fused, cs);
     const int er = (int)hipStreamEndCapture(cs, &b->graph);
     rc = lr ? lr : er;
   }
@@ -6063,11 +6095,11 @@ int ogs_hip_batch_destroy(ogs_hip_batch_t batch) {
 
 int ogs_hip_batch_info(ogs_hip_batch_t batch, ogs_hip_batch_info_t *info) {
   if (!batch || !info) return OGS_E_ARG;
-  info->items = batch->tables.total();
+  info->items = batch->tables.total() + batch->planItems;
   int nodes = 0, shaped = 0;
   for (int c = 0; c < ogs_hip::kPipeClasses; ++c) shaped += batch->tables.n[c] > 0;
   for (int c = ogs_hip::kPipeClasses; c < ogs_hip::kPipeClasses + 2; ++c) nodes += batch->tables.n[c] > 0;
-  info->nodes = batch->cfg.plans ? (int)batch->nplans : nodes + (batch->cfg.fused ? (shaped > 0) : shaped);
+  info->nodes = batch->nplans + nodes + (batch->cfg.fused ? (shaped > 0) : shaped);
   info->plans = batch->cfg.plans;
   info->narrow = batch->cfg.narrow;
   info->order = batch->cfg.order;
