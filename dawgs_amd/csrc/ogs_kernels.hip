@@ -3883,25 +3883,40 @@ static const DeltaPack *byte_kv_pack(const Plan *p, const Partition **pt) {
   return &d;
 }
 
-template <typename T, int OP>
-static void launch_byte_kv_op(const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
+// Field counts of the byte-coded K-field kernels: up to 24 bytes of fields
+// per entry (against the entry-major kernels at 48^3 p=7: fp64 K = 2
+// GatherScatter 389 -> 355 us, Gather 304 -> 295 us; float K = 4 389 ->
+// 355 us; fp64 K = 4, 32 bytes, 722 -> 736 us and Gather 570 -> 648 us, not
+// taken).
+template <typename T>
+constexpr bool byte_k(int K) { return K == 2 || K == 3 || (K == 4 && sizeof(T) <= 4); }
+
+template <typename T, int OP, int KT>
+static void launch_byte_kv_k(const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
   const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
-  hipLaunchKernelGGL((k_gs_sym_byte_kv<T, OP, 3>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, d.f.pk, d.f.base,
+  hipLaunchKernelGGL((k_gs_sym_byte_kv<T, OP, KT>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, d.f.pk, d.f.base,
                      d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E, tb, ncols, v,
                      (pt.tiled & kTiledGS) ? pt.order : nullptr, reinterpret_cast<const u32x2 *>(d.bw),
                      reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
 }
 
+template <typename T, int OP>
+static void launch_byte_kv_op(int K, const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
+  if (K == 2) launch_byte_kv_k<T, OP, 2>(pt, d, ncols, v, st);
+  else if (K == 3) launch_byte_kv_k<T, OP, 3>(pt, d, ncols, v, st);
+  else if constexpr (sizeof(T) <= 4) launch_byte_kv_k<T, OP, 4>(pt, d, ncols, v, st);
+}
+
 template <typename T>
-static int launch_byte_kv(int op, const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
+static int launch_byte_kv(int op, int K, const Partition &pt, const DeltaPack &d, int ncols, T *v, hipStream_t st) {
   switch (op) {
-    case OGS_ADD: launch_byte_kv_op<T, OGS_ADD>(pt, d, ncols, v, st); break;
+    case OGS_ADD: launch_byte_kv_op<T, OGS_ADD>(K, pt, d, ncols, v, st); break;
     case OGS_MUL:
-      if constexpr (kOffered<T, OGS_MUL>) launch_byte_kv_op<T, OGS_MUL>(pt, d, ncols, v, st);
+      if constexpr (kOffered<T, OGS_MUL>) launch_byte_kv_op<T, OGS_MUL>(K, pt, d, ncols, v, st);
       else return OGS_E_NOT_SUPPORTED;
       break;
-    case OGS_MAX: launch_byte_kv_op<T, OGS_MAX>(pt, d, ncols, v, st); break;
-    case OGS_MIN: launch_byte_kv_op<T, OGS_MIN>(pt, d, ncols, v, st); break;
+    case OGS_MAX: launch_byte_kv_op<T, OGS_MAX>(K, pt, d, ncols, v, st); break;
+    case OGS_MIN: launch_byte_kv_op<T, OGS_MIN>(K, pt, d, ncols, v, st); break;
     default: return OGS_E_ARG;
   }
   return (int)hipGetLastError();
@@ -3947,8 +3962,8 @@ static int launch_gs_typed(const Plan *p, int op, int trans, int K, void *d_v, h
       // 579 -> 328 us; bfloat16, whose values are widened, stays on the
       // entry-major kernel: 253 -> 282 us at 48^3 p=7)
       const Partition *bp = nullptr;
-      const DeltaPack *bd = K == 3 ? byte_kv_pack(p, &bp) : nullptr;
-      if (bd) return launch_byte_kv<T>(op, *bp, *bd, p->Ncols, v, st);
+      const DeltaPack *bd = byte_k<T>(K) ? byte_kv_pack(p, &bp) : nullptr;
+      if (bd) return launch_byte_kv<T>(op, K, *bp, *bd, p->Ncols, v, st);
     }
     if (vp) return launch_vec<T>(op, K, vp, list_view(p, gListN, vp, false), gN, v, st);
   }
@@ -4130,27 +4145,34 @@ static bool gather_kv_ready(const Plan *p) {
   return d.w && d.bw && d.seg8 && d.csr == 0 && p->partT[0].nblocks >= kFillPerCu * p->cus;
 }
 
-template <typename T, int OP>
-static void launch_gather_kv_op(const Plan *p, const T *v, T *gv, hipStream_t st) {
+template <typename T, int OP, int KT>
+static void launch_gather_kv_k(const Plan *p, const T *v, T *gv, hipStream_t st) {
   const Partition &pt = p->partT[0];
   const DeltaPack &d = p->dpack;
   const uint4 tb = make_uint4(d.tbl[0], d.tbl[1], d.tbl[2], d.tbl[3]);
-  hipLaunchKernelGGL((k_gather_byte_kv<T, OP, 3>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, d.f.pk, d.f.base,
+  hipLaunchKernelGGL((k_gather_byte_kv<T, OP, KT>), dim3(pt.nblocks), dim3(kBlock), 0, st, pt.d, d.f.pk, d.f.base,
                      d.f.stride, reinterpret_cast<const u32x4 *>(d.w), d.x, d.E, tb, p->Ncols, v, gv,
                      (pt.tiled & kTiledGather) ? pt.order : nullptr, reinterpret_cast<const u32x2 *>(d.bw),
                      reinterpret_cast<const uint4 *>(d.bh), reinterpret_cast<const u32x2 *>(d.bx), d.bt);
 }
 
+template <typename T, int OP>
+static void launch_gather_kv_op(int K, const Plan *p, const T *v, T *gv, hipStream_t st) {
+  if (K == 2) launch_gather_kv_k<T, OP, 2>(p, v, gv, st);
+  else if (K == 3) launch_gather_kv_k<T, OP, 3>(p, v, gv, st);
+  else if constexpr (sizeof(T) <= 4) launch_gather_kv_k<T, OP, 4>(p, v, gv, st);
+}
+
 template <typename T>
-static int launch_gather_kv(int op, const Plan *p, const T *v, T *gv, hipStream_t st) {
+static int launch_gather_kv(int op, int K, const Plan *p, const T *v, T *gv, hipStream_t st) {
   switch (op) {
-    case OGS_ADD: launch_gather_kv_op<T, OGS_ADD>(p, v, gv, st); break;
+    case OGS_ADD: launch_gather_kv_op<T, OGS_ADD>(K, p, v, gv, st); break;
     case OGS_MUL:
-      if constexpr (kOffered<T, OGS_MUL>) launch_gather_kv_op<T, OGS_MUL>(p, v, gv, st);
+      if constexpr (kOffered<T, OGS_MUL>) launch_gather_kv_op<T, OGS_MUL>(K, p, v, gv, st);
       else return OGS_E_NOT_SUPPORTED;
       break;
-    case OGS_MAX: launch_gather_kv_op<T, OGS_MAX>(p, v, gv, st); break;
-    case OGS_MIN: launch_gather_kv_op<T, OGS_MIN>(p, v, gv, st); break;
+    case OGS_MAX: launch_gather_kv_op<T, OGS_MAX>(K, p, v, gv, st); break;
+    case OGS_MIN: launch_gather_kv_op<T, OGS_MIN>(K, p, v, gv, st); break;
     default: return OGS_E_ARG;
   }
   return (int)hipGetLastError();
@@ -4170,8 +4192,8 @@ static int launch_gather_typed(const Plan *p, int op, int trans, int K, void *d_
     if (vp && K == 1 && vp == &p->partT[0] && p->dpack.w)
       return launch_gather_delta<T>(op, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv), st);
     if constexpr (kSeg8<T> || std::is_same<T, f32c>::value) {
-      if (K == 3 && gather_kv_ready(p))
-        return launch_gather_kv<T>(op, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv), st);
+      if (byte_k<T>(K) && gather_kv_ready(p))
+        return launch_gather_kv<T>(op, K, p, static_cast<const T *>(d_v), static_cast<T *>(d_gv), st);
     }
     if (vp) {
       const ListView G = list_view(p, false, vp, false);

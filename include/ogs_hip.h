@@ -180,9 +180,10 @@ int ogs_hip_extract(int type, int32_t N, int32_t K, const int32_t *d_ids,
  * chunks that read the int32 map.  byteItems, byteFallback: exception items
  * of the K=1 GatherScatter's one-byte codes over the delta pack's blocks
  * (both 0 and deltaE > 0: the 16-bit words serve), and their blocks that run
- * from the 16-bit words.  byteK3: 1 where the one-list GatherScatter with
- * three fields of float, double, int32, int64 or float-compensated values
- * runs over those codes' blocks too (rows of at most 8 members, no block left in CSR order, blocks
+ * from the 16-bit words.  byteK3: 1 where the one-list GatherScatter and
+ * the Gather with 2 or 3 fields of float, double, int32, int64 or
+ * float-compensated values (4 fields of the 4-byte ones) run over those
+ * codes' blocks too (rows of at most 8 members, no block left in CSR order, blocks
  * enough to fill the device).  bytes:
  * device memory the plan owns (partitions, packs, column maps, block
  * orders), beside the caller's CSR. */

@@ -229,3 +229,38 @@ def test_byte_gather_k3_bundled_plan(torch, D):
     g, ref, N = mesh(D, (128, 128, 128, 2), fields=D.PlanK1)
     assert g.plan.info()["bundleTy"] > 0
     run_gather(torch, D, g, ref, N, np.float64, O.ADD, 80)
+
+
+@pytest.mark.parametrize("k", [2, 4])
+@pytest.mark.parametrize("ty", list(TYPES))
+@pytest.mark.parametrize("op", [0, 3])
+def test_byte_k2_k4(torch, D, k, ty, op):
+    """Two and four fields over the same codes (four fields of 8-byte values
+    on the entry-major kernels): GatherScatter Sym and Gather Trans,
+    bit-exact."""
+    g, ref, N = mesh(D, (16, 16, 16, 7))
+    dt = TYPES[ty]
+    rng = np.random.default_rng(hash((k, ty, op, "k24")) % 2 ** 32)
+    v = values(rng, dt, k * N)
+    want = ref.gather_scatter(v.copy(), k, op, O.SYM)
+    d = torch.from_numpy(v.copy()).cuda()
+    g.GatherScatter(d, k, op, D.Sym)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), want), ("gs", k, np.dtype(dt).name, op)
+    wg = ref.gather(v, k, op, O.TRANS)
+    gq = torch.zeros(len(wg), dtype=d.dtype, device="cuda")
+    g.plan.Gather(gq, torch.from_numpy(v.copy()).cuda(), k, op, D.Trans)
+    torch.cuda.synchronize()
+    assert same(gq.cpu().numpy(), wg), ("gather", k, np.dtype(dt).name, op)
+
+
+@pytest.mark.parametrize("k", [2, 4])
+def test_byte_k2_k4_float_c(torch, D, k):
+    g, ref, N = mesh(D, (16, 16, 16, 7))
+    rng = np.random.default_rng(200 + k)
+    v = (rng.uniform(-1, 1, k * N) * 10.0 ** rng.uniform(-3, 3, k * N)).astype(np.float32)
+    want = ref.gather_scatter(v.copy(), k, O.ADD, O.SYM, type=O.FLOAT_C)
+    d = torch.from_numpy(v.copy()).cuda()
+    g.GatherScatter(d, k, O.ADD, D.Sym, type=D.FloatC)
+    torch.cuda.synchronize()
+    assert same(d.cpu().numpy(), want)
