@@ -4259,11 +4259,14 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   const T *gv = static_cast<const T *>(d_gv);
   T *v = static_cast<T *>(d_v);
   const RowMap *rm = useN ? &p->rowMapN : &p->rowMapT;
-  // (Row-wise over partition 0's byte codes -- the block's rows staged, each
-  // entry's row from the head counts, ~1 index byte per entry -- measured
-  // slower where no row-delta map serves: fp64 192^3 p=1 150 -> 272 us,
-  // 128^3 p=2 171 -> 293 us, 96^3 p=3 183 -> 191 us; a block's entries write
-  // each element's line in parts, which other blocks write too.)
+  // (Row-wise over partition 0's byte codes -- the block's rows staged in
+  // LDS, each entry's row from per-group head counts, ~1 index byte per
+  // entry -- measured slower: fp64 K = 1 192^3 p=1 150 -> 272 us, 128^3
+  // p=2 171 -> 293 us, 96^3 p=3 183 -> 191 us, where a block's entries
+  // write each element's line in parts that other blocks write too (the
+  // by-column kernel writes whole lines); fp64 K = 3 48^3 p=7 533 -> 942 us,
+  // 96^3 p=3 432 -> 783 us, with 54 KiB of LDS per 2,048-entry block
+  // against the row-wise kernel's 512-entry blocks.)
   if ((colRow || (K == 1 && rm->codes)) && (int64_t)K * (int64_t)sizeof(T) <= 16)
     return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st, rm, p->NrowsT);
   const bool partN = useN && !p->aliased;
