@@ -1766,15 +1766,33 @@ __device__ __forceinline__ void segk_fold(const uint8_t *__restrict__ s, int f, 
     part[k] = c;
   }
   __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) x[k] = sn[k];
   // the last own row on to group t + 1's first head
   const int fh = __builtin_ctz(bits >> 16);
+  if constexpr (OPEN) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    A y = c;
-    R::apply(y, x[k]);
-    c = k < fh ? y : c;
+    for (int k = 0; k < 8; ++k) x[k] = sn[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      A y = c;
+      R::apply(y, x[k]);
+      c = k < fh ? y : c;
+    }
+  } else {
+    // only as many of group t + 1's positions as some lane of the wave
+    // still needs (a row of at most 8 members continues by at most 7; at
+    // p = 7 mostly by none or one)
+    int fm = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) fm += __builtin_amdgcn_ballot_w64(fh > k) != 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      if (k < fm) {
+        const L xk = sn[k];
+        A y = c;
+        R::apply(y, xk);
+        c = k < fh ? y : c;
+      }
+    }
   }
   res[7] = R::fin(c);
 #pragma unroll
