@@ -1725,7 +1725,7 @@ __host__ __device__ constexpr int segk_group() { return KT * 8 * (int)sizeof(L) 
 // result of the row holding position 8t + k, from the group's first head on;
 // with OPEN also before it, the row open at 8t folded again from group
 // t - 1's last head (bits 0 .. 7: group t - 1's head bits, all set for
-// t = 0).  OPEN serves the plain types' three-field kernel (fp64 K = 3 at
+// t = 0).  OPEN serves the plain types' K-field GatherScatter (fp64 K = 3 at
 // 48^3 p=7: 560 us, against 572 us folding each row once); the compensated
 // sums, whose adds cost more, fold each row once.
 template <typename T, int OP, int KT, bool OPEN>
@@ -1851,7 +1851,7 @@ __device__ __forceinline__ void byte_kv_decode(int b, int n, const uint32_t *__r
   }
 }
 
-// The block's 3 fields per slot loaded and staged as groups of 8 positions
+// The block's KT fields per slot loaded and staged as groups of 8 positions
 // (KT x 8 values, field-major, then the 8 head bytes); ends with a barrier.
 template <typename T, int KT>
 __device__ __forceinline__ void byte_kv_stage(const uint32_t (&col)[8], const uint32_t (&w2)[4], int n,
@@ -1950,8 +1950,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_sym_byte_kv(
 }
 
 // Gather with KT fields over the byte pack's blocks (Trans/Sym, and NoTrans
-// of an Unsigned operator; rows of at most 8 members): the three-field
-// kernel's decode and stage, each row folded once by the thread holding its
+// of an Unsigned operator; rows of at most 8 members): the K-field
+// GatherScatter's decode and stage, each row folded once by the thread holding its
 // head, the block's rows' results put in row order in the stage (row index
 // = heads before it: per thread, a wave scan, the waves before through
 // wsum) and stored as one contiguous run, gq[(r0 + row) * KT + f].
@@ -3888,7 +3888,7 @@ static int launch_delta(int op, const Plan *p, T *v, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
-// The one-list GatherScatter with three fields over the byte codes (the
+// The one-list GatherScatter with K fields over the byte codes (the
 // bundled blocks where the plan has them, else partition 0): for lists whose
 // rows have at most 8 members, with no block left in CSR order, and blocks
 // enough to fill the device (else null).
@@ -4156,7 +4156,7 @@ static int launch_gather_delta(int op, const Plan *p, const T *v, T *gv, hipStre
   return (int)hipGetLastError();
 }
 
-// Gather with three fields over partition 0's byte codes (rows of at most 8
+// Gather with K fields over partition 0's byte codes (rows of at most 8
 // members, no block left in CSR order, blocks enough to fill the device).
 static bool gather_kv_ready(const Plan *p) {
   const DeltaPack &d = p->dpack;
