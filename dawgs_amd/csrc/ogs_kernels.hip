@@ -4285,7 +4285,14 @@ static int launch_scatter_typed(const Plan *p, int trans, int K, void *d_v, cons
   // by-column kernel writes whole lines); fp64 K = 3 48^3 p=7 533 -> 942 us,
   // 96^3 p=3 432 -> 783 us, with 54 KiB of LDS per 2,048-entry block
   // against the row-wise kernel's 512-entry blocks.)
-  if ((colRow || (K == 1 && rm->codes)) && (int64_t)K * (int64_t)sizeof(T) <= 16)
+  // Fields of up to 24 bytes go by columns too on lists of at least 3
+  // entries per row (low degree), where the row-wise kernels' partial lines
+  // cost most: fp64 K = 3 192^3 p=1 756 -> 482 us, 128^3 p=2 646 -> 541 us
+  // (96^3 p=3, whose elements fill whole lines, 432 -> 522 us by columns:
+  // row-wise there).
+  const int64_t colBytes = (int64_t)K * (int64_t)sizeof(T);
+  const bool colsFit = colBytes <= 16 || (colBytes <= 24 && (int64_t)(useN ? p->nnzN : p->nnzT) >= 3 * (int64_t)nrows);
+  if ((colRow || (K == 1 && rm->codes)) && colsFit)
     return launch_scatter_cols<T>(p->Ncols, K, colRow, gv, v, st, rm, p->NrowsT);
   const bool partN = useN && !p->aliased;
   // K = 1 from a sparse T list without empty rows: rows staged in LDS, each

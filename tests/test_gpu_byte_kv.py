@@ -264,3 +264,22 @@ def test_byte_k2_k4_float_c(torch, D, k):
     g.GatherScatter(d, k, O.ADD, D.Sym, type=D.FloatC)
     torch.cuda.synchronize()
     assert same(d.cpu().numpy(), want)
+
+
+# --------------------------- Scatter, three fields by columns at low degree
+@pytest.mark.parametrize("box", [(104, 100, 96, 1), (80, 64, 64, 2)], ids=["104x100x96_p1", "80x64x64_p2"])
+@pytest.mark.parametrize("dt", [np.float64, np.int64, np.float32])
+def test_scatter_k3_low_degree(torch, D, box, dt):
+    """Scatter with three fields on lists of at least 3 entries per row (by
+    columns for fields of up to 24 bytes): NoTrans, and Trans of the
+    Unsigned operator, bit-exact."""
+    g, ref, N = mesh(D, box)
+    rng = np.random.default_rng(hash((box, np.dtype(dt).name, "s3")) % 2 ** 32)
+    for trans in (O.NOTRANS, O.TRANS):
+        gv = values(rng, dt, K * ref.op.NrowsT)
+        base = values(rng, dt, K * N)
+        want = ref.scatter(base.copy(), gv, K, trans)
+        d = torch.from_numpy(base.copy()).cuda()
+        g.plan.Scatter(d, torch.from_numpy(gv.copy()).cuda(), K, trans)
+        torch.cuda.synchronize()
+        assert same(d.cpu().numpy(), want), (box, np.dtype(dt).name, trans)
